@@ -1,0 +1,69 @@
+"""ctypes binding of the HIP engine library (C ABI of include/altopir_engine.h).
+
+The library is built in-tree by `python -m altopir_public_amd.build`.  Loading fails
+loudly when it is missing: there is no CPU fallback for the answer engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libaltopir_hip.so"
+
+# every entry point declared in include/altopir_engine.h
+SYMBOLS = (
+    "sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build",
+    "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
+    "alto_answer_device", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
+)
+
+_lib = None
+
+P = C.c_void_p
+SZ = C.c_size_t
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"HIP engine library {LIB_PATH} is not built; run `python -m altopir_public_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.sw_amx_init.restype = C.c_int
+    lib.sw_amx_init.argtypes = []
+    lib.sw_alloc_zeroed.restype = P
+    lib.sw_alloc_zeroed.argtypes = [SZ]
+    lib.sw_alloc.restype = P
+    lib.sw_alloc.argtypes = [SZ]
+    lib.sw_free.restype = None
+    lib.sw_free.argtypes = [P]
+    lib.sw_tile_db.restype = None
+    lib.sw_tile_db.argtypes = [P, P, SZ, SZ, SZ, C.c_int]
+    lib.sw_pack_build.restype = None
+    lib.sw_pack_build.argtypes = [P, P, P, P, P, SZ, SZ, SZ, SZ, C.c_uint64, C.c_int]
+    lib.alto_engine_new.restype = P
+    lib.alto_engine_new.argtypes = [P, P, SZ, SZ, SZ, P, SZ, SZ, P, SZ, SZ, C.c_uint64, C.c_int, P, P,
+                                    C.c_uint64, C.c_int]
+    lib.alto_engine_free.restype = None
+    lib.alto_engine_free.argtypes = [P]
+    lib.alto_request_len.restype = SZ
+    lib.alto_request_len.argtypes = [P]
+    lib.alto_response_len.restype = SZ
+    lib.alto_response_len.argtypes = [P]
+    lib.alto_answer.restype = C.c_int
+    lib.alto_answer.argtypes = [P, P, P, SZ, P, P, C.c_int, P]
+    lib.alto_answer_device.restype = C.c_int
+    lib.alto_answer_device.argtypes = [P, P, SZ, SZ, P, SZ, P, P]
+    lib.alto_engine_device_bytes.restype = SZ
+    lib.alto_engine_device_bytes.argtypes = [P]
+    lib.alto_last_error.restype = C.c_char_p
+    lib.alto_last_error.argtypes = []
+    lib.alto_last_timings.restype = C.c_int
+    lib.alto_last_timings.argtypes = [P, P, C.c_int]
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().alto_last_error().decode(errors="replace")

@@ -1,0 +1,493 @@
+// alto_engine.hip -- host side of the MI355X answer engine: the reference's C ABI
+// (src/engine.rs:4-20, src/amx_kernels.cpp:97-615) over HBM-resident setup state and the
+// kernels of alto_kernels.hpp.
+#include <hip/hip_runtime.h>
+#include <sys/mman.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/altopir_engine.h"
+#include "alto_kernels.hpp"
+
+using namespace alto;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define HIP_OK(expr)                                                                   \
+    ([&]() -> bool {                                                                   \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+            return false;                                                              \
+        }                                                                              \
+        return true;                                                                   \
+    }())
+
+uint64_t host_mulmod(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) % kQ); }
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ workspace --
+
+struct Workspace {
+    size_t cap = 0;              // queries (padded to 4)
+    size_t slot_stride = 0;
+    uint8_t *slots = nullptr;    // host-API request arena on the device
+    uint8_t *out = nullptr;      // host-API response arena on the device
+    int8_t *qa = nullptr;        // scan limbs, cap * 4 * rows bytes
+    int8_t *ya = nullptr;        // packing limbs, cap * 4 * KP bytes
+    uint32_t *acc = nullptr;     // cap * C
+    uint32_t *beta = nullptr;    // cap * C
+    uint32_t *qsum = nullptr;    // cap
+    unsigned long long *ysum = nullptr;
+    int *status = nullptr;       // cap
+    void release() {
+        for (void *p : {(void *)slots, (void *)out, (void *)qa, (void *)ya, (void *)acc, (void *)beta, (void *)qsum,
+                        (void *)ysum, (void *)status})
+            if (p) (void)hipFree(p);
+        *this = Workspace();
+    }
+};
+
+struct Engine {
+    int device = 0;
+    size_t ld = 0, rows = 0, cols = 0, n = 0, tg = 0, kp = 0, outputs = 0;
+    size_t part_len = 0, part_words = 0, row1_off = 0;
+    uint64_t q1 = 0;
+    int q1_bits = 0, threads = 1, logn = 0;
+    bool tail_ok = false;
+    size_t KC = 0, KCp = 0, CT = 0;
+    int8_t *db = nullptr;        // CT x KC blocks
+    int8_t *u = nullptr;         // CT x KCp x 4 blocks
+    uint32_t *ucorr = nullptr;   // cols
+    uint32_t *that = nullptr;    // outputs x kp
+    uint32_t *wtab = nullptr;    // n
+    uint8_t *prefix = nullptr;   // outputs x part_words * 4
+    size_t device_bytes = 0;
+    uint32_t k128e = 0, ycorr0 = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[7] = {};  // start, validate, scan prep, scan, pack prep, pack, tail
+    bool timed_valid = false;
+    Workspace ws;
+    std::vector<int> hstatus;
+
+    ~Engine() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        ws.release();
+        for (void *p : {(void *)db, (void *)u, (void *)ucorr, (void *)that, (void *)wtab, (void *)prefix})
+            if (p) (void)hipFree(p);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t request_len() const { return 9 + 4 * (2 * kp + rows); }
+    size_t response_len() const { return outputs * part_len; }
+
+    // Grows the per-batch buffers to hold `B` queries; host_arenas also sizes the request
+    // and response arenas of the synchronous entry point.
+    bool reserve(size_t B, bool host_arenas) {
+        const size_t Bp = round_up(std::max<size_t>(B, 1), 4);
+        const bool grow = Bp > ws.cap;
+        const bool add_host = host_arenas && !ws.slots;
+        if (!grow && !add_host) return true;
+        (void)hipStreamSynchronize(stream);
+        const size_t cap = grow ? round_up(std::max(Bp, ws.cap + ws.cap / 2), 4) : ws.cap;
+        const bool with_host = host_arenas || ws.slots;
+        ws.release();
+        ws.cap = cap;
+        ws.slot_stride = round_up(ALTO_FRAME_OFFSET + request_len(), 256);
+        bool ok = HIP_OK(hipMalloc(&ws.qa, cap * 4 * rows)) && HIP_OK(hipMalloc(&ws.ya, cap * 4 * kp)) &&
+                  HIP_OK(hipMalloc(&ws.acc, cap * cols * 4)) && HIP_OK(hipMalloc(&ws.beta, cap * cols * 4)) &&
+                  HIP_OK(hipMalloc(&ws.qsum, cap * 4)) && HIP_OK(hipMalloc(&ws.ysum, cap * 8)) &&
+                  HIP_OK(hipMalloc(&ws.status, cap * 4));
+        if (ok && with_host)
+            ok = HIP_OK(hipMalloc(&ws.slots, cap * ws.slot_stride)) && HIP_OK(hipMalloc(&ws.out, cap * response_len()));
+        if (!ok) ws.release();
+        return ok;
+    }
+
+    int enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out, size_t out_stride, int *status,
+                hipStream_t st, bool timed, bool record_start = true);
+};
+
+// Scan launch shapes: (MB, NB) query/column tiles per wave.
+template <int MB, int NB>
+void launch_scan(const Engine &e, size_t MT, size_t B, hipStream_t st) {
+    const int gx = (int)((e.CT + 4 * NB - 1) / (4 * NB));
+    const int gy = (int)((MT + MB - 1) / MB);
+    // split the record dimension until the chip holds ~2 waves of workgroups
+    int gz = 1;
+    while ((size_t)gx * gy * gz < 2048 && e.KC / (size_t)(gz * 2) >= 16) gz *= 2;
+    const int grid = 8 * ((gx + 7) / 8) * gy * gz;
+    (void)B;
+    if (gz > 1)
+        hipLaunchKernelGGL((scan_kernel<MB, NB, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.qa,
+                           (const i32x4 *)e.db, e.ws.acc, (int)e.KC, (int)e.CT, (int)MT, gx, gy, gz, (int)e.cols);
+    else
+        hipLaunchKernelGGL((scan_kernel<MB, NB, false>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.qa,
+                           (const i32x4 *)e.db, e.ws.acc, (int)e.KC, (int)e.CT, (int)MT, gx, gy, gz, (int)e.cols);
+}
+
+template <int MB, int NB>
+void launch_pack(const Engine &e, size_t MT, hipStream_t st) {
+    const int gx = (int)((e.CT + 4 * NB - 1) / (4 * NB));
+    const int gy = (int)((MT + MB - 1) / MB);
+    const int grid = 8 * ((gx + 7) / 8) * gy;
+    hipLaunchKernelGGL((pack_kernel<MB, NB>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.ya,
+                       (const i32x4 *)e.u, e.ucorr, e.ws.ysum, e.ws.beta, (int)e.KCp, (int)e.CT, (int)MT, gx, gy,
+                       (int)e.cols, e.k128e, e.ycorr0);
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+int Engine::enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out, size_t out_stride, int *status,
+                    hipStream_t st, bool timed, bool record_start) {
+    const size_t Bp = round_up(B, 4), MT = Bp / 4;
+    if (timed && record_start) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, slots, stride, status, (uint32_t)kp,
+                       ws.qsum, ws.ysum);
+    if (timed) (void)hipEventRecord(ev[1], st);
+    // scan: query limbs, then the contraction into ws.acc
+    hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0, st,
+                       slots, stride, (size_t)(2 * kp), (const int *)status, (int)B, KC, ws.qa, ws.qsum, ws.ysum);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, ws.acc, Bp * cols, 0u);
+    if (timed) (void)hipEventRecord(ev[2], st);
+    if (MT == 1) launch_scan<1, 4>(*this, MT, B, st);
+    else if (MT == 2) launch_scan<2, 4>(*this, MT, B, st);
+    else launch_scan<4, 4>(*this, MT, B, st);
+    if (timed) (void)hipEventRecord(ev[3], st);
+    // packing: key limbs, then the contraction into ws.beta
+    hipLaunchKernelGGL((limb_prep_kernel<false>), dim3((unsigned)((KCp * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
+                       st, slots, stride, (size_t)0, (const int *)status, (int)B, KCp, ws.ya, ws.qsum, ws.ysum);
+    if (timed) (void)hipEventRecord(ev[4], st);
+    if (MT == 1) launch_pack<1, 1>(*this, MT, st);
+    else if (MT == 2) launch_pack<2, 1>(*this, MT, st);
+    else launch_pack<4, 1>(*this, MT, st);
+    if (timed) (void)hipEventRecord(ev[5], st);
+    TailArgs ta;
+    ta.slots = slots;
+    ta.stride = stride;
+    ta.status = status;
+    ta.acc = ws.acc;
+    ta.qsum = ws.qsum;
+    ta.beta = ws.beta;
+    ta.that = that;
+    ta.wtab = wtab;
+    ta.prefix = prefix;
+    ta.out = out;
+    ta.out_stride = out_stride;
+    ta.n = (int)n;
+    ta.logn = logn;
+    ta.tg = (int)tg;
+    ta.C = (int)cols;
+    ta.kp = (int)kp;
+    ta.part_len = (int)part_len;
+    ta.part_words = (int)part_words;
+    ta.row1_off = (long long)row1_off;
+    ta.q1_bits = q1_bits;
+    ta.q1 = (uint32_t)q1;
+    const size_t lds = (n + part_words) * 4;
+    hipLaunchKernelGGL(tail_kernel, dim3((unsigned)outputs, (unsigned)B), dim3(256), lds, st, ta);
+    if (timed) (void)hipEventRecord(ev[6], st);
+    timed_valid = timed;
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        set_err("kernel launch failed: %s", hipGetErrorString(le));
+        return -1;
+    }
+    return 0;
+}
+
+bool tail_config_ok(size_t n, uint64_t q1, int q1_bits, size_t row1_off, size_t part_len) {
+    // sw_intt_q (src/amx_kernels.cpp:463) and sw_switch_row1 (src/amx_kernels.cpp:517)
+    if (n < 16 || (n & (n - 1))) return false;
+    if (q1 >= (1ull << 21) || n % 8 || (row1_off + n * (size_t)q1_bits + 7) / 8 > part_len || q1_bits > 24) return false;
+    return true;
+}
+
+}  // namespace
+
+// ======================================================================== C ABI ==
+
+extern "C" {
+
+const char *alto_last_error(void) { return g_err.c_str(); }
+
+int sw_amx_init(void) { return 0; }
+
+// 2 MiB-aligned anonymous mapping with its extent stored just before the pointer.
+struct MapHdr {
+    void *base;
+    size_t len;
+};
+static void *map_aligned(size_t bytes) {
+    const size_t al = 2u << 20, body = round_up(std::max<size_t>(bytes, 1), al), len = body + 2 * al;
+    void *base = mmap(nullptr, len, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    if (base == MAP_FAILED) return nullptr;
+    uint8_t *p = (uint8_t *)round_up((uintptr_t)base + sizeof(MapHdr), al);
+    ((MapHdr *)p)[-1] = MapHdr{base, len};
+    return p;
+}
+void *sw_alloc(size_t bytes) { return map_aligned(bytes); }
+void *sw_alloc_zeroed(size_t bytes) { return map_aligned(bytes); /* fresh anonymous pages are zero */ }
+void sw_free(void *p) {
+    if (!p) return;
+    MapHdr h = ((MapHdr *)p)[-1];
+    munmap(h.base, h.len);
+}
+
+static void parallel_for(size_t n, int threads, const std::function<void(size_t, size_t)> &fn) {
+    threads = std::max(1, std::min<int>(threads, (int)std::max<size_t>(n, 1)));
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) th.emplace_back([&, t] { fn(n * t / threads, n * (t + 1) / threads); });
+    for (auto &x : th) x.join();
+}
+
+void sw_tile_db(uint8_t *dst, const uint8_t *db, size_t ld, size_t k, size_t m, int threads) {
+    const size_t KC = k / 64;
+    parallel_for(m / 16, threads, [&](size_t lo, size_t hi) {
+        for (size_t cb = lo; cb < hi; cb++)
+            for (size_t r = 0; r < 16; r++)
+                for (size_t kc = 0; kc < KC; kc++)
+                    memcpy(dst + ((cb / 2) * KC + kc) * 2048 + (cb % 2) * 1024 + r * 64, db + (cb * 16 + r) * ld + kc * 64,
+                           64);
+    });
+}
+
+void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *const *bold_t_bar,
+                   const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg, size_t n, size_t rho,
+                   uint64_t qmod, int threads) {
+    const size_t K = tg * n, KC = K / 64, CT = rho * n / 16;
+    parallel_for(rho * n, threads, [&](size_t lo, size_t hi) {
+        std::vector<uint64_t> col(K);
+        for (size_t oz = lo; oz < hi; oz++) {
+            const size_t o = oz / n, z = oz % n;
+            std::fill(col.begin(), col.end(), 0);
+            for (size_t i = 0; i < num_iter; i++) {
+                const uint32_t g = gen_pows[i];
+                const size_t zp = tables[((g - 1) / 2) * n + z], zb = tables[((2 * n - g - 1) / 2) * n + z];
+                const uint64_t *bt = bold_t[o] + i * tg * n + z, *bb = bold_t_bar[o] + i * tg * n + z;
+                for (size_t k = 0; k < tg; k++) {
+                    col[k * n + zp] += bt[k * n];
+                    col[k * n + zb] += bb[k * n];
+                }
+            }
+            const size_t ct = oz / 16, nn = oz % 16;
+            for (size_t kk = 0; kk < K; kk++) {
+                const uint32_t v = (uint32_t)(col[kk] % qmod);
+                const size_t kc = kk / 64, r = (kk % 64) / 4, t = kk % 4;
+                for (size_t j = 0; j < 4; j++)
+                    ub[((j * (CT / 2) + ct / 2) * KC + kc) * 2048 + (ct % 2) * 1024 + r * 64 + 4 * nn + t] =
+                        (int8_t)((int)((v >> (8 * j)) & 0xff) - 128);
+            }
+        }
+    });
+}
+
+void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld, size_t rows, size_t cols,
+                      const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len,
+                      size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
+                      const uint64_t *inv_table, uint64_t q, int threads) {
+    // configuration checks of src/amx_kernels.cpp:551-552 (the L2-size probe has no GPU
+    // meaning), plus empty shapes, which the reference accepts and later aborts on
+    if (!db_col || !db_tiled || !ub || !prefix || !that || !inv_table || q != kQ || !n || n > 4096 || n % 16 ||
+        cols % n || rows % 64 || cols % 32 || threads <= 0 || (tg * n) % 128 || !rows || !cols || !tg) {
+        set_err("unsupported engine configuration");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return nullptr;
+    }
+    auto *e = new Engine;
+    if (!HIP_OK(hipGetDevice(&e->device))) { delete e; return nullptr; }
+    e->ld = ld; e->rows = rows; e->cols = cols; e->n = n; e->tg = tg; e->kp = tg * n; e->outputs = cols / n;
+    e->part_len = part_len; e->part_words = (part_len + 3) / 4; e->row1_off = row1_off; e->q1 = q1;
+    e->q1_bits = q1_bits; e->threads = threads;
+    e->tail_ok = tail_config_ok(n, q1, q1_bits, row1_off, part_len);
+    while ((size_t)1 << e->logn < n) e->logn++;
+    e->KC = rows / 64; e->KCp = e->kp / 64; e->CT = cols / 16;
+    e->k128e = (uint32_t)((128ull * kE) % kQ);
+    e->ycorr0 = (uint32_t)((kQ - host_mulmod(host_mulmod(e->k128e, e->k128e), e->kp % kQ)) % kQ);
+    const size_t lds = (n + e->part_words) * 4;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto &ev : e->ev) ok = ok && HIP_OK(hipEventCreate(&ev));
+    ok = ok && HIP_OK(hipFuncSetAttribute((const void *)tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)std::min<size_t>(lds, 160 * 1024)));
+    if (e->tail_ok && lds > 160 * 1024) e->tail_ok = false;
+    // resident state
+    const size_t dbb = rows * cols, ubb = 4 * e->kp * cols;
+    ok = ok && HIP_OK(hipMalloc(&e->db, dbb)) && HIP_OK(hipMalloc(&e->u, ubb)) &&
+         HIP_OK(hipMalloc(&e->ucorr, cols * 4)) && HIP_OK(hipMalloc(&e->that, e->outputs * e->kp * 4)) &&
+         HIP_OK(hipMalloc(&e->wtab, n * 4)) && HIP_OK(hipMalloc(&e->prefix, e->outputs * e->part_words * 4 + 4));
+    e->device_bytes = dbb + ubb + cols * 4 + e->outputs * e->kp * 4 + n * 4 + e->outputs * e->part_words * 4;
+    // database: column slices through a staging buffer, tiled on the device
+    uint8_t *stage = nullptr;
+    const size_t stage_cols = std::max<size_t>(16, std::min<size_t>(cols, ((size_t)256 << 20) / rows / 16 * 16));
+    const size_t stage_bytes = std::max(stage_cols * rows, e->kp * cols);
+    ok = ok && HIP_OK(hipMalloc(&stage, stage_bytes));
+    for (size_t c0 = 0; ok && c0 < cols; c0 += stage_cols) {
+        const size_t nc = std::min(stage_cols, cols - c0);
+        ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc, hipMemcpyHostToDevice, e->stream));
+        const size_t work = nc / 16 * e->KC * 64;
+        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, e->stream, stage,
+                           (size_t)rows, e->db, nc / 16, e->KC, c0 / 16);
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    // packing matrix: one byte plane at a time
+    for (int j = 0; ok && j < 4; j++) {
+        ok = HIP_OK(hipMemcpyAsync(stage, ub + (size_t)j * e->kp * cols, e->kp * cols, hipMemcpyHostToDevice, e->stream));
+        const size_t work = e->CT * e->KCp * 64;
+        hipLaunchKernelGGL(tile_u_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, e->stream,
+                           (const int8_t *)stage, e->u, e->CT, e->KCp, j);
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    if (stage) (void)hipFree(stage);
+    if (ok) {
+        hipLaunchKernelGGL(u_colsum_kernel, dim3((unsigned)e->CT), dim3(256), 0, e->stream, (const int8_t *)e->u,
+                           e->ucorr, e->KCp, e->k128e);
+        ok = HIP_OK(hipGetLastError());
+    }
+    // tail keys reduced mod Q (the reference reduces them per use, src/amx_kernels.cpp:507),
+    // twiddles as the low 32 bits the reference multiplies with (_mm512_mul_epu32), prefixes
+    // zero-padded to whole words
+    if (ok) {
+        std::vector<uint32_t> th(e->outputs * e->kp), w(n), pre(e->outputs * e->part_words, 0);
+        for (size_t o = 0; o < e->outputs; o++)
+            for (size_t i = 0; i < e->kp; i++) th[o * e->kp + i] = (uint32_t)(that[o][i] % kQ);
+        for (size_t i = 0; i < n; i++) w[i] = (uint32_t)(inv_table[i] & 0xffffffffull);
+        for (size_t o = 0; o < e->outputs; o++) memcpy((uint8_t *)pre.data() + o * e->part_words * 4, prefix[o], part_len);
+        ok = HIP_OK(hipMemcpy(e->that, th.data(), th.size() * 4, hipMemcpyHostToDevice)) &&
+             HIP_OK(hipMemcpy(e->wtab, w.data(), n * 4, hipMemcpyHostToDevice)) &&
+             HIP_OK(hipMemcpy(e->prefix, pre.data(), pre.size() * 4, hipMemcpyHostToDevice)) &&
+             HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    if (!ok) {
+        const std::string msg = g_err;
+        delete e;
+        g_err = msg;
+        return nullptr;
+    }
+    return e;
+}
+
+void alto_engine_free(void *p) { delete (Engine *)p; }
+
+size_t alto_request_len(const void *p) { return ((const Engine *)p)->request_len(); }
+size_t alto_response_len(const void *p) { return ((const Engine *)p)->response_len(); }
+size_t alto_engine_device_bytes(const void *p) { return ((const Engine *)p)->device_bytes; }
+
+int alto_answer_device(void *p, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out, size_t out_stride,
+                       int *status, void *stream) {
+    auto *e = (Engine *)p;
+    if (!e || !slots || !out || !status || ((uintptr_t)slots & 15) || (slot_stride & 15) ||
+        slot_stride < ALTO_FRAME_OFFSET + e->request_len() || out_stride < e->response_len()) {
+        set_err("alto_answer_device: bad arguments");
+        return -1;
+    }
+    if (!B) return 0;
+    if (!e->tail_ok) {
+        set_err("unsupported tail configuration");
+        return -3;
+    }
+    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    if (!e->reserve(B, false)) return -4;
+    hipStream_t st = stream ? (hipStream_t)stream : e->stream;
+    return e->enqueue(slots, slot_stride, B, out, out_stride, status, st, true);
+}
+
+int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B, uint8_t *const *out, int *status,
+                int method, double *stage_ms) {
+    auto *e = (Engine *)p;
+    if (!e || !req || !len || !out || !status ||
+        (method != ALTO_METHOD_AMX && method != ALTO_METHOD_VNNI && method != ALTO_METHOD_HIP)) {
+        set_err("alto_answer: bad arguments or method");
+        return -1;
+    }
+    if (stage_ms) std::fill(stage_ms, stage_ms + 4, 0.0);
+    if (!B) return 0;
+    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    if (!e->reserve(B, true)) return -1;
+    const size_t want = e->request_len(), rl = e->response_len(), S = e->ws.slot_stride;
+    e->hstatus.assign(B, 0);
+    for (size_t c = 0; c < B; c++) e->hstatus[c] = (!req[c] || len[c] != want) ? 1 : 0;
+    hipStream_t st = e->stream;
+    bool ok = HIP_OK(hipEventRecord(e->ev[0], st));
+    for (size_t c = 0; ok && c < B; c++)
+        if (!e->hstatus[c])
+            ok = HIP_OK(hipMemcpyAsync(e->ws.slots + c * S + ALTO_FRAME_OFFSET, req[c], want, hipMemcpyHostToDevice, st));
+    ok = ok && HIP_OK(hipMemcpyAsync(e->ws.status, e->hstatus.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!ok) return -1;
+    if (!e->tail_ok) {
+        // validation only: the reference fails the whole batch with -3 once any request is valid
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, e->ws.slots, S, e->ws.status,
+                           (uint32_t)e->kp, e->ws.qsum, e->ws.ysum);
+        if (!HIP_OK(hipMemcpyAsync(status, e->ws.status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
+            !HIP_OK(hipStreamSynchronize(st)))
+            return -1;
+        for (size_t c = 0; c < B; c++)
+            if (status[c] == 0) return -3;
+        return 0;
+    }
+    // the upload is timed as part of the validation stage (ev[0] was recorded before it)
+    if (e->enqueue(e->ws.slots, S, B, e->ws.out, rl, e->ws.status, st, true, false)) return -1;
+    if (!HIP_OK(hipMemcpyAsync(status, e->ws.status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
+        !HIP_OK(hipStreamSynchronize(st)))
+        return -1;
+    int answered = 0;
+    for (size_t c = 0; c < B; c++)
+        if (status[c] == 0) {
+            answered++;
+            if (!HIP_OK(hipMemcpyAsync(out[c], e->ws.out + c * rl, rl, hipMemcpyDeviceToHost, st))) return -1;
+        }
+    // the download is timed as part of the tail stage
+    if (!HIP_OK(hipEventRecord(e->ev[6], st)) || !HIP_OK(hipStreamSynchronize(st))) return -1;
+    if (stage_ms) {
+        // {validate, scan, pack, tail} as in src/amx_kernels.cpp:613
+        const int from[4] = {0, 1, 3, 5}, to[4] = {1, 3, 5, 6};
+        for (int i = 0; i < 4; i++) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e->ev[from[i]], e->ev[to[i]]);
+            stage_ms[i] = ms;
+        }
+    }
+    return answered;
+}
+
+int alto_last_timings(void *p, double *ms, int count) {
+    auto *e = (Engine *)p;
+    if (!e || !ms || count <= 0 || !e->timed_valid) return 0;
+    if (!HIP_OK(hipEventSynchronize(e->ev[6]))) return -1;
+    const int k = std::min(count, 6);
+    for (int i = 0; i < k; i++) {
+        float f = 0;
+        (void)hipEventElapsedTime(&f, e->ev[i], e->ev[i + 1]);
+        ms[i] = f;
+    }
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Seeded engine setup inputs and request frames for tests and benchmarks.
+
+The answer engine treats its setup artefacts -- the database, the packing matrix U, the
+tail keys `that`, the inverse-NTT table and the row-0 prefixes -- as opaque inputs
+(src/amx_kernels.cpp:546-560), so any seeded values exercise the same arithmetic as the
+offline precomputation of a real server.  Shapes and the tail layout follow the
+reference's parameters: q1 = 2^10, q2 = 2^18, row 1 starts at bit 18 * n, a part is
+ceil((18 + 10) * n / 64) * 8 bytes (src/service.rs:164-167, src/modulus_switch.rs:136-147).
+
+Requests follow the wire format of src/service.rs:178-198: flag byte 0, the two key
+lengths (u32 LE, = tg * n), y and z (u32 LE, each < Q), then one u32 query word per record.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import synthetic
+
+Q = synthetic.Q
+
+
+def part_len_for(n: int, q1_bits: int = 10, q2_bits: int = 18) -> int:
+    """Bytes of one response part (src/modulus_switch.rs:139-140)."""
+    return ((q2_bits * n + q1_bits * n + 63) // 64) * 64 // 8
+
+
+def request_len(rows: int, kp: int) -> int:
+    return 9 + 4 * (2 * kp + rows)
+
+
+def bitrev(x: int, bits: int) -> int:
+    r = 0
+    for _ in range(bits):
+        r = (r << 1) | (x & 1)
+        x >>= 1
+    return r
+
+
+def primitive_root_2n(n: int) -> int:
+    """Smallest primitive 2n-th root of unity mod Q (2n must divide Q - 1 = 2^12 * 1048573)."""
+    assert (Q - 1) % (2 * n) == 0
+    for g in range(2, 1 << 16):
+        psi = pow(g, (Q - 1) // (2 * n), Q)
+        if pow(psi, n, Q) == Q - 1:
+            return psi
+    raise ValueError("no root")
+
+
+def inverse_table(n: int, psi: int | None = None) -> np.ndarray:
+    """Twiddles for the Gentleman-Sande inverse transform of the engine:
+    table[k] = psi^-bitrev(k) / 2 mod Q, so that log2(n) halvings give the 1/n factor."""
+    psi = primitive_root_2n(n) if psi is None else psi
+    logn = n.bit_length() - 1
+    psi_inv = pow(psi, Q - 2, Q)
+    inv2 = (Q + 1) // 2
+    t = np.zeros(n, dtype=np.uint64)
+    for k in range(1, n):
+        t[k] = pow(psi_inv, bitrev(k, logn), Q) * inv2 % Q
+    return t
+
+
+@dataclass
+class EngineInputs:
+    rows: int
+    cols: int
+    n: int
+    tg: int
+    ld: int
+    db_col: np.ndarray           # uint8, cols * ld (column-major, ld >= rows)
+    ub: np.ndarray               # int8, 4 * tg * n * cols (reference tile layout)
+    prefixes: list = field(default_factory=list)  # cols / n uint8 arrays of part_len
+    that: list = field(default_factory=list)      # cols / n uint64 arrays of tg * n
+    inv_table: np.ndarray = None  # uint64, n
+    part_len: int = 0
+    row1_off: int = 0
+    q1: int = 1024
+    q1_bits: int = 10
+
+    @property
+    def kp(self) -> int:
+        return self.tg * self.n
+
+    @property
+    def outputs(self) -> int:
+        return self.cols // self.n
+
+    @property
+    def request_len(self) -> int:
+        return request_len(self.rows, self.kp)
+
+    @property
+    def response_len(self) -> int:
+        return self.outputs * self.part_len
+
+    def db_record_major(self) -> np.ndarray:
+        """The database as records x bytes (row k = record k)."""
+        return self.db_col.reshape(self.cols, self.ld)[:, : self.rows].T
+
+
+def make_inputs(rows: int, cols: int, n: int = 2048, tg: int = 4, seed: int = 0, ld: int | None = None,
+                table: str = "ntt", prefix: str = "row0", db: np.ndarray | None = None) -> EngineInputs:
+    """Seeded setup artefacts.  db_col byte (col * ld + k) is byte (col * rows + k) of the
+    stream `seed` (i.e. byte `col` of record `k`); U's tile bytes come from stream seed+1,
+    `that` from seed+2 (full 64-bit values), random inverse tables from seed+3, prefixes
+    from seed+4.  table="ntt" gives the real twiddles of a 2n-th root of unity,
+    table="random" arbitrary u64 values (only their low 32 bits matter).  prefix="row0"
+    fills the row-0 bits and leaves row 1 zero as a server does; prefix="random" fills
+    every byte, which exercises the OR of row-1 fields onto the prefix."""
+    ld = rows if ld is None else ld
+    kp = tg * n
+    if db is None:
+        body = synthetic.uniform_bytes(seed, rows * cols).reshape(cols, rows)
+        if ld == rows:
+            db_col = body.reshape(-1)
+        else:
+            db_col = np.zeros((cols, ld), dtype=np.uint8)
+            db_col[:, :rows] = body
+            db_col = db_col.reshape(-1)
+    else:
+        db_col = db
+    ub = synthetic.uniform_bytes(seed + 1, 4 * kp * cols).view(np.int8)
+    outputs = cols // n
+    th = synthetic.stream(seed + 2, outputs * kp).reshape(outputs, kp)
+    if table == "ntt" and (Q - 1) % (2 * n) == 0:
+        inv = inverse_table(n)
+    else:
+        inv = synthetic.stream(seed + 3, n)
+    plen = part_len_for(n)
+    row1_off = 18 * n
+    pre = synthetic.uniform_bytes(seed + 4, outputs * plen).reshape(outputs, plen).copy()
+    if prefix == "row0":
+        pre[:, row1_off // 8:] = 0
+        if row1_off % 8:
+            pre[:, row1_off // 8] &= (1 << (row1_off % 8)) - 1
+    return EngineInputs(rows=rows, cols=cols, n=n, tg=tg, ld=ld, db_col=db_col, ub=ub,
+                        prefixes=[pre[o] for o in range(outputs)], that=[th[o] for o in range(outputs)],
+                        inv_table=inv, part_len=plen, row1_off=row1_off)
+
+
+def encode_request(y: np.ndarray, z: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """Wire frame of src/service.rs:178-198 (as a uint8 array)."""
+    kp = len(y)
+    assert len(z) == kp
+    out = np.empty(9 + 4 * (2 * kp + len(q)), dtype=np.uint8)
+    out[0] = 0
+    out[1:5] = np.frombuffer(np.uint32(kp).tobytes(), dtype=np.uint8)
+    out[5:9] = out[1:5]
+    body = np.concatenate([np.asarray(y, dtype="<u4"), np.asarray(z, dtype="<u4"), np.asarray(q, dtype="<u4")])
+    out[9:] = body.view(np.uint8)
+    return out
+
+
+def make_requests(B: int, rows: int, kp: int, seed: int) -> list[np.ndarray]:
+    """B valid frames: y, z uniform in [0, Q) and q uniform u32, from stream seed*1000003 + c."""
+    reqs = []
+    for c in range(B):
+        s = seed * 1000003 + c
+        y = synthetic.uniform_below_q(s * 3 + 1, kp)
+        z = synthetic.uniform_below_q(s * 3 + 2, kp)
+        q = synthetic.uniform_u32(s * 3 + 3, rows)
+        reqs.append(encode_request(y, z, q))
+    return reqs
+
+
+def fill_requests(arena: np.ndarray, B: int, rows: int, kp: int, seed: int) -> None:
+    """Writes make_requests(B, ...) frames into rows of a (B, >= request_len) uint8 arena."""
+    L = request_len(rows, kp)
+    for c, f in enumerate(make_requests(B, rows, kp, seed)):
+        arena[c, :L] = f

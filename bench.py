@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X answer engine: QPS of whole batches with the requests resident
+in HBM, the roofline of the dominant kernel and the reference CPU engine beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--records R] [--batch B]
+
+A step is one batch through the device entry point (validate -> scan -> packing -> tail)
+with the frames already in device memory.  Default workload: 1 GB database (32,768
+records x 32 KB, bytes uniform from seed 0) and a batch of 128 requests (y, z uniform in
+[0, Q), query words uniform u32, seed 2), the reference's default parameters (n = 2048,
+tg = 4, q1 = 2^10, q2 = 2^18).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "QPS at 1/8/64 GB (batch 1 & 128) + p99 latency; scan GB/s & MFMA util vs peak"
+HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def round_up(x, m):
+    return (x + m - 1) // m * m
+
+
+def cpu_baseline(inp, reqs, threads: int, budget_s: float) -> dict:
+    """The reference engine compiled from its own source (oracle/_ref) on the host cores,
+    method VNNI (AVX-512) or AMX, timed like `altopir bench` (src/bin/altopir.rs:741-826):
+    one warm-up, then trials until the budget is spent (at most 5), median."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ctypes as oc
+
+    B = len(reqs)
+    if oc.ref_available() and oc.ref_methods():
+        method = 2 if 2 in oc.ref_methods() else 0
+        ref = oc.RefEngine(inp, threads=threads)
+        out = np.zeros((B, inp.response_len), np.uint8)
+        st = np.full(B, -1, np.int32)
+        ref.answer(reqs, out, st, method)  # warm-up
+        times = []
+        t_end = time.time() + budget_s
+        while len(times) < 5 and (not times or time.time() + times[-1] < t_end):
+            t = time.perf_counter()
+            r, _ = ref.answer(reqs, out, st, method)
+            times.append(time.perf_counter() - t)
+            assert r == B
+        ref.close()
+        med = statistics.median(times)
+        return {"value": round(B / med, 3), "unit": "queries/s", "cores": threads, "kind": "reference",
+                "sample": f"reference engine (src/amx_kernels.cpp, method {'vnni' if method == 2 else 'amx'}) "
+                          f"on the same {B}-request batch, {len(times)} trials after a warm-up, median "
+                          f"{med * 1e3:.1f} ms"}
+    # no reference build or no AVX-512 VNNI/AMX: time the oracle port on one request
+    out = np.zeros((1, inp.response_len), np.uint8)
+    st = np.full(1, -1, np.int32)
+    t = time.perf_counter()
+    oc.oracle_answer(inp, reqs[:1], out, st)
+    dt = time.perf_counter() - t
+    return {"value": round(1 / dt, 4), "unit": "queries/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"oracle restatement, 1 request of the workload, {dt:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--records", type=int, default=32768)
+    ap.add_argument("--record-bytes", type=int, default=32768)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from altopir_public_amd import workload
+    from altopir_public_amd.engine import Engine
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    R, C, B = args.records, args.record_bytes, args.batch
+    n, tg = 2048, 4
+    t0 = time.time()
+    inp = workload.make_inputs(R, C, n=n, tg=tg, seed=0)
+    log(f"[bench] inputs {R} x {C} B generated in {time.time() - t0:.1f} s")
+    t0 = time.time()
+    eng = Engine.from_inputs(inp, threads=args.cpu_threads)
+    log(f"[bench] engine set up in {time.time() - t0:.1f} s, {eng.device_bytes() / 2**30:.2f} GiB resident")
+    L, rl = inp.request_len, inp.response_len
+    stride = round_up(7 + L, 256)
+    reqs = workload.make_requests(B, R, inp.kp, seed=2)
+    host = np.zeros((B, stride), np.uint8)
+    for c, f in enumerate(reqs):
+        host[c, 7:7 + L] = f
+    slots = torch.from_numpy(host).to("cuda")
+    status = torch.zeros(B, dtype=torch.int32, device="cuda")
+    out = torch.empty((B, rl), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        rc = eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(), stream)
+        if rc != 0:
+            raise RuntimeError(f"alto_answer_device returned {rc}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    assert int((status != 0).sum()) == 0, "a valid request was rejected"
+    log("[bench] warm-up done")
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    end.record()
+    torch.cuda.synchronize()
+    ms_per_step = start.elapsed_time(end) / args.steps
+    qps = B / (ms_per_step / 1e3)
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {qps:.1f} QPS")
+
+    # per-kernel device times (HIP events recorded by the engine around each launch)
+    stages = []
+    for _ in range(min(args.steps, 10)):
+        step()
+        stages.append(eng.last_timings())
+    st = np.array(stages).mean(axis=0)
+    names = ["validate", "scan_prep", "scan", "pack_prep", "pack", "tail"]
+    stage_ms = {k: round(float(v), 4) for k, v in zip(names, st)}
+    log(f"[bench] stage ms {stage_ms}")
+
+    KP = tg * n
+    scan_ops, pack_ops = 8 * R * C * B, 32 * KP * C * B
+    scan_bytes, pack_bytes = C * R + 4 * B * R, 4 * KP * C + 4 * B * KP
+    dom = "scan" if stage_ms["scan"] >= stage_ms["pack"] else "pack"
+    t_dom = stage_ms[dom] / 1e3
+    ops, byts = (scan_ops, scan_bytes) if dom == "scan" else (pack_ops, pack_bytes)
+    # ridge: 5.0e15 op/s / 8.0e12 B/s = 625 op/B
+    ridge_bound = "mfma" if ops / byts > 625.0 else "hbm"
+    if ridge_bound == "mfma":
+        achieved = ops / t_dom / 1e12
+        roof = {"bound": "mfma", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
+                "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                "algorithmic_ops": ops}
+    else:
+        achieved = byts / t_dom / 1e9
+        roof = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": byts}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(inp, reqs, args.cpu_threads, args.cpu_budget)
+        log(f"[bench] cpu baseline {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{R * C / 2**30:g} GiB database ({R} records x {C} B, uniform bytes seed 0), "
+                        f"batch {B} (uniform requests seed 2), requests resident in HBM",
+            "records": R, "record_bytes": C, "batch": B, "n": n, "tg": tg, "parallelism": "single GPU",
+        },
+        "stage_ms": stage_ms,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,114 @@
+/*
+ * altopir_engine.h -- C ABI of the MI355X answer engine.
+ *
+ * The first eleven entry points are the reference engine's ABI, with the same
+ * signatures, argument meaning, status codes and error returns, so that the
+ * reference's Rust wrapper (src/engine.rs:4-20) links against this library
+ * unchanged.  Each declaration names the reference interface it replaces.
+ *
+ * Differences in behaviour that a caller can observe:
+ *   - alto_engine_new copies (uploads) every setup buffer into HBM; the caller may
+ *     release its host copies afterwards (the reference borrows them).
+ *   - method: 0 (amx) and 2 (vnni) are accepted for compatibility and, like
+ *     ALTO_METHOD_HIP, run the HIP engine; anything else returns -1.
+ *   - on -3 (unsupported tail configuration) no output buffer is written.
+ *
+ * Everything below "Extensions" has no counterpart in the reference.
+ */
+#ifndef ALTOPIR_ENGINE_H
+#define ALTOPIR_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALTO_METHOD_AMX 0  /* src/engine.rs:44-49 */
+#define ALTO_METHOD_VNNI 2 /* src/engine.rs:44-49 */
+#define ALTO_METHOD_HIP 4  /* new: the HIP engine under its own name */
+
+/* replaces sw_amx_init, src/amx_kernels.cpp:151 (src/engine.rs:5).  Returns 0: the GPU
+ * engine needs no CPU tile state. */
+int sw_amx_init(void);
+
+/* replace sw_alloc_zeroed / sw_alloc / sw_free, src/amx_kernels.cpp:97-113
+ * (src/engine.rs:6-8).  Host memory, 2 MiB aligned; nullptr on failure. */
+void *sw_alloc_zeroed(size_t bytes);
+void *sw_alloc(size_t bytes);
+void sw_free(void *p);
+
+/* replaces sw_tile_db, src/amx_kernels.cpp:138-149 (src/engine.rs:9).  Writes the same
+ * 16x64 tile order as the reference so that any consumer of the tiled copy keeps
+ * working; this engine itself uploads from db_col. */
+void sw_tile_db(uint8_t *dst, const uint8_t *db, size_t ld, size_t k, size_t m, int threads);
+
+/* replaces sw_pack_build, src/amx_kernels.cpp:311-341 (src/engine.rs:10-11).  Builds the
+ * packing matrix in the reference's byte-plane tile layout (setup work, on `threads`
+ * host threads). */
+void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *const *bold_t_bar,
+                   const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg,
+                   size_t n, size_t rho, uint64_t qmod, int threads);
+
+/* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
+ * Returns nullptr for the configurations the reference rejects (Q != 4294955009,
+ * n == 0 or n > 4096 or n % 16, cols % n, rows % 64, cols % 32, threads <= 0,
+ * (tg * n) % 128), for rows == 0, cols == 0 or tg == 0, and when no GPU or not
+ * enough device memory is available. */
+void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld, size_t rows,
+                      size_t cols, const int8_t *ub, size_t n, size_t tg,
+                      const uint8_t *const *prefix, size_t part_len, size_t row1_off, uint64_t q1,
+                      int q1_bits, const uint64_t *const *that, const uint64_t *inv_table,
+                      uint64_t q, int threads);
+
+/* replaces alto_engine_free, src/amx_kernels.cpp:561 (src/engine.rs:15). */
+void alto_engine_free(void *e);
+
+/* replace alto_request_len / alto_response_len, src/amx_kernels.cpp:562-563
+ * (src/engine.rs:16-17): 9 + 4 * (2 * tg * n + rows) and (cols / n) * part_len. */
+size_t alto_request_len(const void *e);
+size_t alto_response_len(const void *e);
+
+/* replaces alto_answer, src/amx_kernels.cpp:564-615 (src/engine.rs:18-19).
+ * Synchronous.  status[c]: 1 null frame or wrong length, 2 flag or key-length field,
+ * 3 key word >= Q, 0 answered.  Only answered outputs are written.  Returns the number
+ * answered, -1 on bad arguments or method, -3 on an unsupported tail configuration.
+ * stage_ms (optional) receives {validate, scan, pack, tail} in milliseconds (device
+ * time of each stage; request upload is counted in validate, response download in
+ * tail).  Not re-entrant on one engine. */
+int alto_answer(void *e, const uint8_t *const *req, const size_t *len, size_t B,
+                uint8_t *const *out, int *status, int method, double *stage_ms);
+
+/* ---------------------------------------------------------------- Extensions -- */
+
+/* Device-resident batch.  Frame c lies at slots + c * slot_stride + ALTO_FRAME_OFFSET in
+ * device memory (slots 16-byte aligned, slot_stride a multiple of 16 and at least
+ * ALTO_FRAME_OFFSET + request_len); every frame is request_len bytes long.  status is a
+ * device array of B ints: on entry 0 for a frame to check, nonzero for a frame the
+ * caller already rejected (it is left as is); on return the answer statuses.  Output c
+ * is written at out + c * out_stride (device memory) when its status is 0.  Everything
+ * is enqueued on `stream` (a hipStream_t; 0 = the engine's own stream) and nothing is
+ * waited for.  Returns 0, -1 on bad arguments, -3 on an unsupported tail
+ * configuration, -4 when device memory for the batch cannot be allocated. */
+#define ALTO_FRAME_OFFSET 7
+int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
+                       size_t out_stride, int *status, void *stream);
+
+/* Device time in milliseconds of the stages of the last batch (synchronous or device
+ * entry point), waiting for it to finish: {validate, scan limb preparation, scan
+ * contraction, packing limb preparation, packing contraction, tail}.  Writes up to
+ * `count` values and returns how many were written (0 if no batch ran yet). */
+int alto_last_timings(void *e, double *ms, int count);
+
+/* Number of HBM bytes the engine holds for its resident state (database, packing
+ * matrix, tail keys, prefixes). */
+size_t alto_engine_device_bytes(const void *e);
+
+/* Last error message of this thread ("" if none). */
+const char *alto_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

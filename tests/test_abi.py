@@ -1,0 +1,134 @@
+"""CPU checks of the C ABI library: it loads, exports every symbol the public header
+declares, and its host-side entry points (allocation, DB tiling, packing-matrix build)
+match the oracle.  No GPU compute here."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from altopir_public_amd import _lib, synthetic, workload
+import oracle_ctypes as oc
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols() -> set:
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M):
+            if m.group(1) not in ("defined",):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert {"alto_answer", "alto_engine_new", "sw_pack_build", "alto_answer_device"} <= names
+    for s in names:
+        assert hasattr(lib, s), f"{s} declared in include/ but not exported"
+    assert set(_lib.SYMBOLS) == names
+
+
+def test_reference_abi_is_complete():
+    # the eleven entry points of src/engine.rs:4-20
+    ref = {"sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build", "alto_engine_new",
+           "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer"}
+    assert ref <= declared_symbols()
+
+
+def test_engine_new_rejects_what_the_reference_rejects():
+    # src/amx_kernels.cpp:551-552: these configurations return nullptr before any device work
+    lib = _lib.load()
+    one = np.zeros(64, np.uint8)
+    pp = (C.c_void_p * 1)(one.ctypes.data)
+    args = dict(db=one.ctypes.data, tiled=one.ctypes.data, ld=64, rows=64, cols=32, ub=one.ctypes.data, n=16, tg=8,
+                prefix=C.addressof(pp), part_len=64, row1_off=0, q1=1024, q1_bits=10, that=C.addressof(pp),
+                inv=one.ctypes.data, q=workload.Q, threads=1)
+
+    def new(**kw):
+        a = dict(args, **kw)
+        return lib.alto_engine_new(a["db"], a["tiled"], a["ld"], a["rows"], a["cols"], a["ub"], a["n"], a["tg"],
+                                   a["prefix"], a["part_len"], a["row1_off"], a["q1"], a["q1_bits"], a["that"],
+                                   a["inv"], a["q"], a["threads"])
+
+    for bad in [dict(q=workload.Q + 2), dict(n=0), dict(n=8192), dict(n=24), dict(cols=48), dict(rows=96),
+                dict(threads=0), dict(tg=1), dict(db=None), dict(tiled=None), dict(ub=None), dict(inv=None),
+                dict(prefix=None), dict(that=None)]:
+        assert not new(**bad), bad
+    assert _lib.last_error()
+
+
+def test_alloc_is_aligned_and_zeroed():
+    lib = _lib.load()
+    p = lib.sw_alloc_zeroed(5 << 20)
+    assert p and p % (2 << 20) == 0
+    buf = np.frombuffer((C.c_uint8 * (5 << 20)).from_address(p), np.uint8)
+    assert not buf.any()
+    buf[:] = 7
+    lib.sw_free(p)
+    q = lib.sw_alloc(123)
+    assert q and q % (2 << 20) == 0
+    lib.sw_free(q)
+    lib.sw_free(None)
+
+
+def test_tile_db_matches_the_oracle_layout():
+    lib, orc = _lib.load(), oc.oracle()
+    rows, cols, ld = 256, 64, 320
+    db = synthetic.uniform_bytes(4, cols * ld)
+    a = np.zeros(rows * cols, np.uint8)
+    b = np.zeros(rows * cols, np.uint8)
+    lib.sw_tile_db(a.ctypes.data, db.ctypes.data, ld, rows, cols, 3)
+    orc.orc_tile_db(b.ctypes.data, db.ctypes.data, ld, rows, cols)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,tg,rho", [(16, 8, 2), (64, 2, 1)])
+def test_pack_build_matches_the_oracle(n, tg, rho):
+    lib, orc = _lib.load(), oc.oracle()
+    num_iter = n // 2 - 1
+    logn = n.bit_length() - 1
+    br = [workload.bitrev(j, logn) for j in range(n)]
+    tables = np.array([[br[((t * (2 * br[j] + 1)) % (2 * n) - 1) // 2] for j in range(n)]
+                       for t in range(1, 2 * n, 2)], dtype=np.uint32).reshape(-1)
+    gen_pows = np.array([pow(5, i, 2 * n) for i in range(n)], dtype=np.uint32)
+    bt = [synthetic.stream(70 + o, num_iter * tg * n) for o in range(rho)]
+    bb = [synthetic.stream(80 + o, num_iter * tg * n) for o in range(rho)]
+    pt = (C.c_void_p * rho)(*[x.ctypes.data for x in bt])
+    pb = (C.c_void_p * rho)(*[x.ctypes.data for x in bb])
+    size = 4 * tg * n * rho * n
+    u1, u2 = np.zeros(size, np.int8), np.zeros(size, np.int8)
+    lib.sw_pack_build(u1.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
+                      num_iter, tg, n, rho, workload.Q, 3)
+    orc.orc_pack_build(u2.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
+                       num_iter, tg, n, rho)
+    assert np.array_equal(u1, u2)
+
+
+def test_host_mirror_names_follow_the_reference():
+    from altopir_public_amd import engine
+
+    assert engine.Method.parse("amx") == engine.Method.AMX and int(engine.Method.AMX) == 0
+    assert engine.Method.parse("vnni") == engine.Method.VNNI and int(engine.Method.VNNI) == 2
+    assert engine.Method.parse("nope") is None
+    for name in ("MappedBytes", "TiledDb", "build_packing_matrix", "Engine", "amx_available", "vnni_available",
+                 "init_amx"):
+        assert hasattr(engine, name)
+    m = engine.MappedBytes.zeroed(4096)
+    assert not m.as_i8().any()
+
+
+def test_synthetic_record_follows_the_reference_generator():
+    # src/service.rs:54-62, computed here with the scalar generator
+    state = (synthetic.SEED ^ ((synthetic.ROW_MUL * 6) & ((1 << 64) - 1)))
+    want = bytearray()
+    for _ in range(8):
+        v, state = synthetic.mix(state)
+        want += int(v % 65535).to_bytes(2, "little")
+    assert bytes(synthetic.record(5, 16)) == bytes(want)

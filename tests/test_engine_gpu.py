@@ -1,0 +1,176 @@
+"""Parity of the HIP answer engine with the oracle, through the C ABI.
+
+Restates tests/engine.rs for the HIP engine: byte-identical responses on the reference's
+shapes (ROWS = 2048 and 8192, RECORD_BYTES = 4096, batches {1, 2, 72, 136}), the
+rejection statuses of hand-made and fuzzed frames with the untouched-buffer rule, plus
+shapes and edge cases the reference's tests do not cover (small rings, ld > rows, batch
+sizes that are not multiples of the tile width, random prefixes).  The oracle
+(oracle/liboracle.so) is pinned against the reference engine in test_oracle.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from altopir_public_amd import workload
+from altopir_public_amd.engine import Engine, Method
+import oracle_ctypes as oc
+import refmodel
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(inp, reqs, method=Method.HIP, sentinel=0):
+    B = len(reqs)
+    rl = inp.response_len
+    eng = Engine.from_inputs(inp)
+    out_g = np.full((B, rl), sentinel, np.uint8)
+    st_g = np.full(B, -1, np.int32)
+    answered, stage = eng.answer(reqs, out_g, st_g, method)
+    out_o = np.full((B, rl), sentinel, np.uint8)
+    st_o = np.full(B, -1, np.int32)
+    r_o = oc.oracle_answer(inp, reqs, out_o, st_o)
+    eng.close()
+    return (answered, st_g, out_g, stage), (r_o, st_o, out_o)
+
+
+def assert_same(g, o, label=""):
+    (a, sg, og, _), (r, so, oo) = g, o
+    assert a == r, f"{label}: answered {a} vs {r}"
+    assert np.array_equal(sg, so), f"{label}: statuses {sg} vs {so}"
+    bad = [i for i in range(len(og)) if not np.array_equal(og[i], oo[i])]
+    assert not bad, f"{label}: responses {bad[:8]} differ from the oracle"
+
+
+@pytest.fixture(scope="module")
+def small_inputs():
+    # tests/engine.rs:10-11: ROWS = 2048, RECORD_BYTES = 4096 (two outputs)
+    return workload.make_inputs(2048, 4096, n=2048, tg=4, seed=1)
+
+
+def test_responses_match_the_oracle(hip_device, small_inputs):
+    inp = small_inputs
+    reqs = workload.make_requests(5, inp.rows, inp.kp, seed=21)
+    g, o = run_both(inp, reqs)
+    assert g[0] == 5 and (g[1] == 0).all()
+    assert_same(g, o, "rows 2048")
+
+
+@pytest.mark.parametrize("method", [Method.AMX, Method.VNNI, Method.HIP])
+def test_every_method_code_runs_the_hip_engine(hip_device, small_inputs, method):
+    inp = small_inputs
+    reqs = workload.make_requests(3, inp.rows, inp.kp, seed=22)
+    g, o = run_both(inp, reqs, method=method)
+    assert_same(g, o, f"method {int(method)}")
+
+
+def test_malformed_requests_are_rejected_like_the_decoder_rejects_them(hip_device, small_inputs):
+    inp = small_inputs
+    r0, r1 = workload.make_requests(2, inp.rows, inp.kp, seed=23)
+    frames, want = refmodel.hand_frames(r0, r1, inp.kp)
+    for f, w in zip(frames, want):
+        assert refmodel.decoder_accepts(f, inp.rows, inp.kp) == (w == 0)
+    g, o = run_both(inp, frames, sentinel=0xA5)
+    assert g[0] == 2
+    assert list(g[1]) == want
+    for c in range(1, 7):
+        assert (g[2][c] == 0xA5).all(), "a rejected buffer was written"
+    assert_same(g, o, "hand frames")
+
+
+def test_fuzzed_frames_follow_the_decoder(hip_device, small_inputs):
+    inp = small_inputs
+    base = workload.make_requests(4, inp.rows, inp.kp, seed=24)
+    eng = Engine.from_inputs(inp)
+    accepted = 0
+    rl = inp.response_len
+    for frames in refmodel.fuzz_batches(base, inp.kp):
+        out = np.zeros((len(frames), rl), np.uint8)
+        st = np.full(len(frames), -1, np.int32)
+        eng.answer(frames, out, st, Method.HIP)
+        for f, s in zip(frames, st):
+            assert (s == 0) == refmodel.decoder_accepts(f, inp.rows, inp.kp), "engine and decoder disagree"
+        accepted += int((st == 0).sum())
+        out_o = np.zeros_like(out)
+        st_o = np.full(len(frames), -1, np.int32)
+        oc.oracle_answer(inp, frames, out_o, st_o)
+        assert np.array_equal(st, st_o)
+        assert np.array_equal(out[st == 0], out_o[st_o == 0])
+    eng.close()
+    assert 0 < accepted < 512
+
+
+@pytest.mark.parametrize("B", [1, 2, 72, 136])
+def test_multi_block_and_chunked_batches_match_the_oracle(hip_device, B):
+    # tests/engine.rs:196-245: ROWS = 8192, batches {1, 2, 72, 136}
+    inp = _block_inputs()
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=100 + B)
+    g, o = run_both(inp, reqs)
+    assert g[0] == B
+    assert_same(g, o, f"B={B}")
+
+
+_BLOCK = None
+
+
+def _block_inputs():
+    global _BLOCK
+    if _BLOCK is None:
+        _BLOCK = workload.make_inputs(8192, 4096, n=2048, tg=4, seed=2)
+    return _BLOCK
+
+
+@pytest.mark.parametrize("rows,cols,n,tg,B,ld,prefix,table", [
+    (64, 32, 16, 8, 3, 64, "random", "random"),       # smallest legal shapes
+    (128, 64, 32, 4, 7, 192, "random", "ntt"),        # ld > rows
+    (2048, 512, 256, 4, 13, 2048, "random", "ntt"),   # 2 outputs of a small ring
+    (4096, 1024, 1024, 2, 9, 4096, "row0", "random"),  # tg = 2
+    (1024, 8192, 4096, 4, 3, 1024, "random", "random"),  # n = 4096 (largest ring)
+])
+def test_shapes_beyond_the_reference_tests(hip_device, rows, cols, n, tg, B, ld, prefix, table):
+    inp = workload.make_inputs(rows, cols, n=n, tg=tg, seed=rows + cols + n, ld=ld, prefix=prefix, table=table)
+    reqs = workload.make_requests(B, rows, inp.kp, seed=B)
+    g, o = run_both(inp, reqs)
+    assert_same(g, o, f"{rows}x{cols} n={n}")
+
+
+def test_extreme_query_and_key_values(hip_device, small_inputs):
+    inp = small_inputs
+    kp, rows = inp.kp, inp.rows
+    reqs = []
+    for yv, zv, qv in [(0, 0, 0), (workload.Q - 1, workload.Q - 1, 0xFFFFFFFF), (1, 2, 1 << 31),
+                       (workload.Q - 1, 0, 0x80808080)]:
+        reqs.append(workload.encode_request(np.full(kp, yv, np.uint32), np.full(kp, zv, np.uint32),
+                                            np.full(rows, qv, np.uint32)))
+    g, o = run_both(inp, reqs)
+    assert g[0] == 4
+    assert_same(g, o, "extreme values")
+
+
+def test_empty_batch_and_null_frames(hip_device, small_inputs):
+    inp = small_inputs
+    eng = Engine.from_inputs(inp)
+    out = np.zeros((0, inp.response_len), np.uint8)
+    assert eng.answer([], out, np.zeros(0, np.int32))[0] == 0
+    r = workload.make_requests(1, inp.rows, inp.kp, seed=5)[0]
+    out = np.full((3, inp.response_len), 7, np.uint8)
+    st = np.full(3, -1, np.int32)
+    a, _ = eng.answer([None, r, b""], out, st)
+    assert a == 1 and list(st) == [1, 0, 1]
+    assert (out[0] == 7).all() and (out[2] == 7).all()
+    eng.close()
+
+
+def test_repeated_batches_of_different_sizes_reuse_the_engine(hip_device, small_inputs):
+    inp = small_inputs
+    eng = Engine.from_inputs(inp)
+    rl = inp.response_len
+    for B, seed in [(9, 1), (2, 2), (17, 3), (1, 4)]:
+        reqs = workload.make_requests(B, inp.rows, inp.kp, seed=300 + seed)
+        out = np.zeros((B, rl), np.uint8)
+        st = np.full(B, -1, np.int32)
+        assert eng.answer(reqs, out, st)[0] == B
+        out_o = np.zeros_like(out)
+        oc.oracle_answer(inp, reqs, out_o, np.full(B, -1, np.int32))
+        assert np.array_equal(out, out_o), f"batch {B}"
+    eng.close()
