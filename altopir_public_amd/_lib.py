@@ -15,6 +15,7 @@ SYMBOLS = (
     "sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build",
     "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
     "alto_answer_device", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
+    "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get",
 )
 
 _lib = None
@@ -61,6 +62,14 @@ def load() -> C.CDLL:
     lib.alto_last_error.argtypes = []
     lib.alto_last_timings.restype = C.c_int
     lib.alto_last_timings.argtypes = [P, P, C.c_int]
+    lib.alto_variant_count.restype = C.c_int
+    lib.alto_variant_count.argtypes = []
+    lib.alto_variant_name.restype = C.c_char_p
+    lib.alto_variant_name.argtypes = [C.c_int]
+    lib.alto_tune_set.restype = C.c_int
+    lib.alto_tune_set.argtypes = [P, C.c_int, SZ, C.c_int]
+    lib.alto_tune_get.restype = C.c_int
+    lib.alto_tune_get.argtypes = [P, C.c_int, SZ]
     _lib = lib
     return lib
 

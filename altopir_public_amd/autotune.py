@@ -1,0 +1,122 @@
+"""Tile-shape tuner for the two contractions of the answer engine.
+
+For a resident engine and a batch size, every kernel variant of the scan and of the
+packing contraction (alto_variant_name) is timed on a device-resident batch with the
+engine's own HIP events (alto_last_timings); the fastest is recorded in the engine's
+tuning table for that batch-size bucket (alto_tune_set).  All variants compute identical
+bytes, so tuning changes only speed.  The table can be saved to and loaded from JSON,
+keyed by (records, record bytes) and bucket.
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib, workload
+
+SCAN, PACK = 0, 1
+
+
+def variant_names() -> list[str]:
+    lib = _lib.load()
+    return [lib.alto_variant_name(i).decode() for i in range(lib.alto_variant_count())]
+
+
+def bucket(B: int) -> int:
+    mt = (B + 3) // 4
+    b = 0
+    while b + 1 < 16 and (1 << b) < mt:
+        b += 1
+    return b
+
+
+class DeviceBatch:
+    """B seeded request frames resident in device memory (torch tensors), in the layout of
+    the device entry point: frame c at slots + c * stride + 7."""
+
+    def __init__(self, engine, inp, B: int, seed: int, stream=None):
+        import torch
+
+        self.B = B
+        self.L = inp.request_len
+        self.rl = inp.response_len
+        self.stride = (7 + self.L + 255) // 256 * 256
+        host = np.zeros((B, self.stride), np.uint8)
+        for c, f in enumerate(workload.make_requests(B, inp.rows, inp.kp, seed=seed)):
+            host[c, 7:7 + self.L] = f
+        self.slots = torch.from_numpy(host).to("cuda")
+        self.status = torch.zeros(B, dtype=torch.int32, device="cuda")
+        self.out = torch.empty((B, self.rl), dtype=torch.uint8, device="cuda")
+        self.engine = engine
+        self.stream = stream
+
+    def run(self) -> None:
+        import torch
+
+        s = self.stream if self.stream is not None else torch.cuda.current_stream().cuda_stream
+        rc = self.engine.answer_device(self.slots.data_ptr(), self.stride, self.B, self.out.data_ptr(), self.rl,
+                                       self.status.data_ptr(), s)
+        if rc != 0:
+            raise RuntimeError(f"alto_answer_device returned {rc} ({_lib.last_error()})")
+
+
+def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict:
+    """Median device time (ms) of the chosen contraction for every variant."""
+    lib = _lib.load()
+    stage = 2 if which == SCAN else 4  # index in alto_last_timings
+    out = {}
+    saved = lib.alto_tune_get(engine.ptr, which, batch.B)
+    for idx, name in enumerate(variant_names()):
+        lib.alto_tune_set(engine.ptr, which, batch.B, idx)
+        ts = []
+        for _ in range(reps + 1):
+            batch.run()
+            ts.append(engine.last_timings()[stage])
+        out[name] = float(np.median(ts[1:]))
+    lib.alto_tune_set(engine.ptr, which, batch.B, saved)
+    return out
+
+
+def tune(engine, inp, batch_sizes, seed: int = 7, reps: int = 3, log=None) -> dict:
+    """Times every variant for each batch size and records the fastest in the engine.
+    Returns {bucket: {"scan": (name, ms), "pack": (name, ms), "B": B}}."""
+    lib = _lib.load()
+    names = variant_names()
+    table = {}
+    for B in batch_sizes:
+        batch = DeviceBatch(engine, inp, B, seed)
+        entry = {"B": B}
+        for which, key in ((SCAN, "scan"), (PACK, "pack")):
+            ms = time_variants(engine, batch, which, reps)
+            best = min(ms, key=ms.get)
+            lib.alto_tune_set(engine.ptr, which, B, names.index(best))
+            entry[key] = (best, ms[best])
+            entry[key + "_all"] = ms
+            if log:
+                log(f"[tune] B={B} {key}: " + ", ".join(f"{k}={v:.3f}" for k, v in sorted(ms.items(), key=lambda x: x[1])))
+        table[bucket(B)] = entry
+    return table
+
+
+def save(table: dict, path: Path, rows: int, cols: int) -> None:
+    data = {}
+    if path.exists():
+        data = json.loads(path.read_text())
+    data[f"{rows}x{cols}"] = {str(k): {"scan": v["scan"][0], "pack": v["pack"][0], "B": v["B"]} for k, v in table.items()}
+    path.write_text(json.dumps(data, indent=1, sort_keys=True))
+
+
+def load(engine, path: Path, rows: int, cols: int) -> int:
+    """Applies a saved table to an engine; returns the number of entries applied."""
+    lib = _lib.load()
+    if not path.exists():
+        return 0
+    names = variant_names()
+    ent = json.loads(path.read_text()).get(f"{rows}x{cols}", {})
+    for _, v in ent.items():
+        for which, key in ((SCAN, "scan"), (PACK, "pack")):
+            if v[key] in names:
+                lib.alto_tune_set(engine.ptr, which, int(v["B"]), names.index(v[key]))
+    return len(ent)

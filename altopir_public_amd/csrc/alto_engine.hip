@@ -45,6 +45,111 @@ uint64_t host_mulmod(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __in
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// ------------------------------------------------------------- launch shapes --
+
+int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Split the k-range so that no workgroup accumulates more than kFoldSteps k-steps in
+// i32, and further (down to min_steps per split) until about `target` workgroups exist.
+int choose_splits(int KC, long long tiles, int target, int min_steps) {
+    int gz = ceil_div(KC, kFoldSteps);
+    while (tiles * gz < target && KC / (gz * 2) >= min_steps) gz *= 2;
+    return gz;
+}
+
+template <int MB, int NBT, int U, bool PACK>
+void launch_stream(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    g.gx = ceil_div(g.NT, 4 * NBT);
+    g.gy = ceil_div(g.MT, MB);
+    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 2048, 16);
+    else g.gz = 1;
+    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    if (g.gz > 1)
+        hipLaunchKernelGGL((stream_kernel<MB, NBT, U, PACK, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
+                           (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((stream_kernel<MB, NBT, U, PACK, false>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
+                           (const i32x4 *)b, g);
+}
+
+template <int TM, int TN, int WM, int WN, bool PACK>
+void launch_lds(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    g.gx = ceil_div(g.NT, WN * TN);
+    g.gy = ceil_div(g.MT, WM * TM);
+    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
+    else g.gz = 1;
+    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    if (g.gz > 1)
+        hipLaunchKernelGGL((lds_kernel<TM, TN, WM, WN, PACK, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
+                           (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((lds_kernel<TM, TN, WM, WN, PACK, false>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
+                           (const i32x4 *)b, g);
+}
+
+template <int TM, int TN, int WM, int WN, int S, bool PACK>
+void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    constexpr int NBLK = WM * TM + WN * TN;
+    constexpr size_t lds = (size_t)S * NBLK * 1024;
+    static bool attr_set = false;  // per instantiation; the attribute is per function
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr_set = true;
+    }
+    g.gx = ceil_div(g.NT, WN * TN);
+    g.gy = ceil_div(g.MT, WM * TM);
+    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
+    else g.gz = 1;
+    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    if (g.gz > 1)
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(256), lds, st,
+                           (const i32x4 *)a, (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false>), dim3(grid), dim3(256), lds, st,
+                           (const i32x4 *)a, (const i32x4 *)b, g);
+}
+
+// Kernel variants of a contraction.  Every variant computes the full result for any number
+// of A tiles (ragged tiles are clamped and masked); they differ in tile shapes and in how
+// operands are staged, which the tuner chooses per batch size.
+typedef void (*LaunchFn)(GemmArgs, const int8_t *, const int8_t *, hipStream_t);
+struct Variant {
+    const char *name;
+    LaunchFn scan, pack;
+};
+#define ALTO_VARIANT(name, ...) {name, __VA_ARGS__<false>, __VA_ARGS__<true>}
+const Variant kVariants[] = {
+    {"stream_m1_n4_u4", launch_stream<1, 4, 4, false>, launch_stream<1, 4, 4, true>},
+    {"stream_m2_n4_u4", launch_stream<2, 4, 4, false>, launch_stream<2, 4, 4, true>},
+    {"stream_m4_n4_u2", launch_stream<4, 4, 2, false>, launch_stream<4, 4, 2, true>},
+    {"stream_m8_n4_u2", launch_stream<8, 4, 2, false>, launch_stream<8, 4, 2, true>},
+    {"lds_m8_n8", launch_lds<8, 8, 2, 2, false>, launch_lds<8, 8, 2, 2, true>},
+    {"dma_m4_n8_s4", launch_dma<4, 8, 2, 2, 4, false>, launch_dma<4, 8, 2, 2, 4, true>},
+    {"dma_m8_n8_s3", launch_dma<8, 8, 2, 2, 3, false>, launch_dma<8, 8, 2, 2, 3, true>},
+    {"dma_m8_n8_s4", launch_dma<8, 8, 2, 2, 4, false>, launch_dma<8, 8, 2, 2, 4, true>},
+};
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+
+// Built-in shape policy (used until the tuner records a choice for a batch-size bucket).
+int default_variant(int MT) {
+    if (MT == 1) return 0;
+    if (MT == 2) return 1;
+    if (MT < 16) return 5;
+    return 6;
+}
+
+// Batch-size buckets of the tuning table: A tiles (4 queries each) 1, 2, 3-4, 5-8, ...,
+// i.e. bucket = ceil(log2(MT)), up to 2^15 tiles.
+constexpr int kBuckets = 16;
+int bucket_of(int MT) {
+    int b = 0;
+    while (b + 1 < kBuckets && (1 << b) < MT) b++;
+    return b;
+}
+
 // ------------------------------------------------------------------ workspace --
 
 struct Workspace {
@@ -88,6 +193,16 @@ struct Engine {
     bool timed_valid = false;
     Workspace ws;
     std::vector<int> hstatus;
+    int tuned[2][kBuckets];  // [scan, pack][bucket] -> variant, -1 = built-in policy
+
+    Engine() {
+        for (auto &t : tuned)
+            for (int &v : t) v = -1;
+    }
+    int variant_for(int which, int MT) const {
+        const int v = tuned[which][bucket_of(MT)];
+        return v >= 0 ? v : default_variant(MT);
+    }
 
     ~Engine() {
         (void)hipSetDevice(device);
@@ -130,34 +245,6 @@ struct Engine {
                 hipStream_t st, bool timed, bool record_start = true);
 };
 
-// Scan launch shapes: (MB, NB) query/column tiles per wave.
-template <int MB, int NB>
-void launch_scan(const Engine &e, size_t MT, size_t B, hipStream_t st) {
-    const int gx = (int)((e.CT + 4 * NB - 1) / (4 * NB));
-    const int gy = (int)((MT + MB - 1) / MB);
-    // split the record dimension until the chip holds ~2 waves of workgroups
-    int gz = 1;
-    while ((size_t)gx * gy * gz < 2048 && e.KC / (size_t)(gz * 2) >= 16) gz *= 2;
-    const int grid = 8 * ((gx + 7) / 8) * gy * gz;
-    (void)B;
-    if (gz > 1)
-        hipLaunchKernelGGL((scan_kernel<MB, NB, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.qa,
-                           (const i32x4 *)e.db, e.ws.acc, (int)e.KC, (int)e.CT, (int)MT, gx, gy, gz, (int)e.cols);
-    else
-        hipLaunchKernelGGL((scan_kernel<MB, NB, false>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.qa,
-                           (const i32x4 *)e.db, e.ws.acc, (int)e.KC, (int)e.CT, (int)MT, gx, gy, gz, (int)e.cols);
-}
-
-template <int MB, int NB>
-void launch_pack(const Engine &e, size_t MT, hipStream_t st) {
-    const int gx = (int)((e.CT + 4 * NB - 1) / (4 * NB));
-    const int gy = (int)((MT + MB - 1) / MB);
-    const int grid = 8 * ((gx + 7) / 8) * gy;
-    hipLaunchKernelGGL((pack_kernel<MB, NB>), dim3(grid), dim3(256), 0, st, (const i32x4 *)e.ws.ya,
-                       (const i32x4 *)e.u, e.ucorr, e.ws.ysum, e.ws.beta, (int)e.KCp, (int)e.CT, (int)MT, gx, gy,
-                       (int)e.cols, e.k128e, e.ycorr0);
-}
-
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -174,17 +261,33 @@ int Engine::enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out,
                        slots, stride, (size_t)(2 * kp), (const int *)status, (int)B, KC, ws.qa, ws.qsum, ws.ysum);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, ws.acc, Bp * cols, 0u);
     if (timed) (void)hipEventRecord(ev[2], st);
-    if (MT == 1) launch_scan<1, 4>(*this, MT, B, st);
-    else if (MT == 2) launch_scan<2, 4>(*this, MT, B, st);
-    else launch_scan<4, 4>(*this, MT, B, st);
+    {
+        GemmArgs g{};
+        g.KC = (int)KC;
+        g.MT = (int)MT;
+        g.NT = (int)CT;
+        g.C = (int)cols;
+        g.out = ws.acc;
+        kVariants[variant_for(0, (int)MT)].scan(g, ws.qa, db, st);
+    }
     if (timed) (void)hipEventRecord(ev[3], st);
     // packing: key limbs, then the contraction into ws.beta
     hipLaunchKernelGGL((limb_prep_kernel<false>), dim3((unsigned)((KCp * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
                        st, slots, stride, (size_t)0, (const int *)status, (int)B, KCp, ws.ya, ws.qsum, ws.ysum);
     if (timed) (void)hipEventRecord(ev[4], st);
-    if (MT == 1) launch_pack<1, 1>(*this, MT, st);
-    else if (MT == 2) launch_pack<2, 1>(*this, MT, st);
-    else launch_pack<4, 1>(*this, MT, st);
+    {
+        GemmArgs g{};
+        g.KC = (int)KCp;
+        g.MT = (int)MT;
+        g.NT = (int)(4 * CT);
+        g.C = (int)cols;
+        g.out = ws.beta;
+        g.ucorr = ucorr;
+        g.ysum = ws.ysum;
+        g.k128e = k128e;
+        g.ycorr0 = ycorr0;
+        kVariants[variant_for(1, (int)MT)].pack(g, ws.ya, u, st);
+    }
     if (timed) (void)hipEventRecord(ev[5], st);
     TailArgs ta;
     ta.slots = slots;
@@ -315,6 +418,11 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
     if (!db_col || !db_tiled || !ub || !prefix || !that || !inv_table || q != kQ || !n || n > 4096 || n % 16 ||
         cols % n || rows % 64 || cols % 32 || threads <= 0 || (tg * n) % 128 || !rows || !cols || !tg) {
         set_err("unsupported engine configuration");
+        return nullptr;
+    }
+    // the packing contraction accumulates all of KP in i32: |y' u'| <= 2^14, so KP <= 2^16
+    if (tg * n > ((size_t)kFoldSteps << 6)) {
+        set_err("unsupported engine configuration: tg * n > 65536");
         return nullptr;
     }
     int ndev = 0;
@@ -475,6 +583,23 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
         }
     }
     return answered;
+}
+
+int alto_variant_count(void) { return kNumVariants; }
+
+const char *alto_variant_name(int idx) { return idx >= 0 && idx < kNumVariants ? kVariants[idx].name : nullptr; }
+
+int alto_tune_set(void *p, int which, size_t B, int idx) {
+    auto *e = (Engine *)p;
+    if (!e || which < 0 || which > 1 || idx < -1 || idx >= kNumVariants || !B) return -1;
+    e->tuned[which][bucket_of((int)(round_up(B, 4) / 4))] = idx;
+    return 0;
+}
+
+int alto_tune_get(const void *p, int which, size_t B) {
+    auto *e = (const Engine *)p;
+    if (!e || which < 0 || which > 1 || !B) return -1;
+    return e->variant_for(which, (int)(round_up(B, 4) / 4));
 }
 
 int alto_last_timings(void *p, double *ms, int count) {

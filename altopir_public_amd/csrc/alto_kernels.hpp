@@ -263,133 +263,325 @@ __device__ __forceinline__ TileMap map_tile(int gx, int gy, int gz) {
     return m;
 }
 
-// Scan contraction.  A = query limbs (MT tiles of 16 rows = 4 queries), B = database
-// (CT tiles of 16 columns), K = KC blocks of 64 records.  Workgroup = 4 waves; wave w
-// computes rows of tiles [y*MB, y*MB+MB) x columns of tiles [(4x+w)*NB, +NB) over the
-// K range of split z, and adds sum_i 256^i P_i (mod 2^32) into acc[c * C + col]
-// (atomically when the K range is split).
-template <int MB, int NB, bool ATOMIC>
-__global__ __launch_bounds__(256) void scan_kernel(const i32x4 *__restrict__ qa, const i32x4 *__restrict__ db,
-                                                   uint32_t *__restrict__ acc_out, int KC, int CT, int MT, int gx,
-                                                   int gy, int gz, int C) {
-    const TileMap tm = map_tile(gx, gy, gz);
-    if (!tm.valid) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ct0 = (tm.x * 4 + wave) * NB, mt0 = tm.y * MB;
-    const int kb = (int)((long long)KC * tm.z / gz), ke = (int)((long long)KC * (tm.z + 1) / gz);
-    const i32x4 *ap[MB];
-    const i32x4 *bp[NB];
-#pragma unroll
-    for (int m = 0; m < MB; m++) ap[m] = qa + (size_t)min(mt0 + m, MT - 1) * KC * 64 + lane;
-#pragma unroll
-    for (int n = 0; n < NB; n++) bp[n] = db + (size_t)min(ct0 + n, CT - 1) * KC * 64 + lane;
-    uint32_t run[MB][NB];
-#pragma unroll
-    for (int m = 0; m < MB; m++)
-#pragma unroll
-        for (int n = 0; n < NB; n++) run[m][n] = 0;
-    for (int k = kb; k < ke;) {
-        const int kend = min(ke, k + kFoldSteps);
-        i32x4 acc[MB][NB];
-#pragma unroll
-        for (int m = 0; m < MB; m++)
-#pragma unroll
-            for (int n = 0; n < NB; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+struct GemmArgs {
+    int KC;        // 64-deep k-steps of the contraction
+    int MT;        // A row tiles (16 rows = 4 queries)
+    int NT;        // B tiles: scan = column tiles, pack = column tiles x 4 byte planes
+    int gx, gy, gz;
+    int C;         // columns (row stride of the outputs)
+    uint32_t *out;                      // scan: acc (u32, += when split), pack: beta (mod Q)
+    const uint32_t *ucorr;              // pack: column corrections
+    const unsigned long long *ysum;     // pack: per-query key sums
+    uint32_t k128e, ycorr0;
+};
 
-        for (; k < kend; k++) {
-            i32x4 a[MB], b[NB];
-#pragma unroll
-            for (int m = 0; m < MB; m++) a[m] = ap[m][(size_t)k * 64];
-#pragma unroll
-            for (int n = 0; n < NB; n++) b[n] = bp[n][(size_t)k * 64];
-#pragma unroll
-            for (int m = 0; m < MB; m++)
-#pragma unroll
-                for (int n = 0; n < NB; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], b[n], acc[m][n], 0, 0, 0);
-        }
-#pragma unroll
-        for (int m = 0; m < MB; m++)
-#pragma unroll
-            for (int n = 0; n < NB; n++)
-                run[m][n] += (uint32_t)acc[m][n].x + ((uint32_t)acc[m][n].y << 8) + ((uint32_t)acc[m][n].z << 16) +
-                             ((uint32_t)acc[m][n].w << 24);
-    }
-#pragma unroll
-    for (int m = 0; m < MB; m++)
-#pragma unroll
-        for (int n = 0; n < NB; n++) {
-            if (mt0 + m >= MT || ct0 + n >= CT) continue;
-            const int c = 4 * (mt0 + m) + (lane >> 4), col = 16 * (ct0 + n) + (lane & 15);
-            if (ATOMIC) atomicAdd(&acc_out[(size_t)c * C + col], run[m][n]);
-            else acc_out[(size_t)c * C + col] = run[m][n];
-        }
+// B operand block of tile t at k-step k (i32x4 units).  Scan: database tile t.  Pack: tile
+// t = 4 * column tile + plane j of the packing matrix.
+template <bool PACK>
+__device__ __forceinline__ const i32x4 *b_block(const i32x4 *b, int t, int KC) {
+    if (PACK) return b + (((size_t)(t >> 2) * KC) * 4 + (t & 3)) * 64;
+    return b + (size_t)t * KC * 64;
+}
+template <bool PACK>
+constexpr int b_kstride() { return PACK ? 256 : 64; }
+
+__device__ __forceinline__ uint32_t recombine_u32(i32x4 a) {
+    return (uint32_t)a.x + ((uint32_t)a.y << 8) + ((uint32_t)a.z << 16) + ((uint32_t)a.w << 24);
 }
 
-// Packing contraction.  A = y limbs (MT tiles), B = the four byte planes of U (block
-// (ct, kc, j)), K = KCp blocks.  Each lane ends with G_ij (i = limb row, j = plane) of one
-// query and one column, recombines them mod Q and adds the corrections:
-//   beta = sum_{ij} 2^{8(i+j)} G_ij + ucorr[col] + ycorr(c)   (mod Q)
-// ycorr(c) = 128E * Ysum[c] - (128E)^2 * KP, the constant passed as ycorr0.
-template <int MB, int NB>
-__global__ __launch_bounds__(256) void pack_kernel(const i32x4 *__restrict__ ya, const i32x4 *__restrict__ ut,
-                                                   const uint32_t *__restrict__ ucorr,
-                                                   const unsigned long long *__restrict__ ysum,
-                                                   uint32_t *__restrict__ beta, int KCp, int CT, int MT, int gx, int gy,
-                                                   int C, uint32_t k128e, uint32_t ycorr0) {
-    const TileMap tm = map_tile(gx, gy, 1);
-    if (!tm.valid) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ct0 = (tm.x * 4 + wave) * NB, mt0 = tm.y * MB;
-    const i32x4 *ap[MB];
-    const i32x4 *bp[NB];
-#pragma unroll
-    for (int m = 0; m < MB; m++) ap[m] = ya + (size_t)min(mt0 + m, MT - 1) * KCp * 64 + lane;
-#pragma unroll
-    for (int n = 0; n < NB; n++) bp[n] = ut + (size_t)min(ct0 + n, CT - 1) * KCp * 256 + lane;
-    i32x4 acc[MB][NB][4];
-#pragma unroll
-    for (int m = 0; m < MB; m++)
-#pragma unroll
-        for (int n = 0; n < NB; n++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[m][n][j] = i32x4{0, 0, 0, 0};
+template <bool ATOMIC>
+__device__ __forceinline__ void scan_store(const GemmArgs &g, int rt, int ct, i32x4 a) {
+    const int lane = threadIdx.x & 63;
+    const int c = 4 * rt + (lane >> 4), col = 16 * ct + (lane & 15);
+    uint32_t *dst = g.out + (size_t)c * g.C + col;
+    if (ATOMIC) atomicAdd(dst, recombine_u32(a));
+    else *dst = recombine_u32(a);
+}
 
-    for (int k = 0; k < KCp; k++) {
-        i32x4 a[MB], b[NB][4];
+// beta = sum_{ij} 2^{8(i+j)} G_ij + ucorr[col] + 128E * Ysum[c] - (128E)^2 KP  (mod Q);
+// a[j] holds G_ij of plane j for the four limb rows i of this lane's query.
+__device__ __forceinline__ void pack_store(const GemmArgs &g, int rt, int ct, const i32x4 *a) {
+    const int lane = threadIdx.x & 63;
+    const int c = 4 * rt + (lane >> 4), col = 16 * ct + (lane & 15);
+    int64_t V[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int m = 0; m < MB; m++) a[m] = ap[m][(size_t)k * 64];
-#pragma unroll
-        for (int n = 0; n < NB; n++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[n][j] = bp[n][(size_t)k * 256 + j * 64];
-#pragma unroll
-        for (int m = 0; m < MB; m++)
-#pragma unroll
-            for (int n = 0; n < NB; n++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[m][n][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], b[n][j], acc[m][n][j], 0, 0, 0);
+    for (int j = 0; j < 4; j++) {
+        V[j + 0] += a[j].x;
+        V[j + 1] += a[j].y;
+        V[j + 2] += a[j].z;
+        V[j + 3] += a[j].w;
     }
     // 2^{8s} mod Q for s = 4, 5, 6 (s <= 3 are exact powers)
     const int64_t w4 = 12287, w5 = 12287ll << 8, w6 = 12287ll << 16;
+    const int64_t t = V[0] + (V[1] << 8) + (V[2] << 16) + (V[3] << 24) + V[4] * w4 + V[5] * w5 + V[6] * w6;
+    const uint32_t yc = addmod(mulmod(red64(g.ysum[c]), g.k128e), g.ycorr0);
+    g.out[(size_t)c * g.C + col] = addmod(addmod(red_i64(t), g.ucorr[col]), yc);
+}
+
+// Streaming contraction for few queries (HBM-bound): operands straight from memory into
+// registers, U k-steps of loads in flight per wave.  Workgroup = 4 waves; wave w of tile
+// (x, y, z) computes A tiles [y*MB, y*MB+MB) x B tiles [(4x+w)*NBT, +NBT) over the k-range
+// of split z (at most kFoldSteps steps, so the i32 accumulators cannot overflow).
+template <int MB, int NBT, int U, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(256) void stream_kernel(const i32x4 *__restrict__ a_src,
+                                                     const i32x4 *__restrict__ b_src, GemmArgs g) {
+    static_assert(!PACK || NBT % 4 == 0, "packing tiles come in groups of four planes");
+    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int bt0 = (tm.x * 4 + wave) * NBT, mt0 = tm.y * MB;
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    constexpr int BS = b_kstride<PACK>();
+    const i32x4 *ap[MB];
+    const i32x4 *bp[NBT];
+#pragma unroll
+    for (int m = 0; m < MB; m++) ap[m] = a_src + (size_t)min(mt0 + m, g.MT - 1) * g.KC * 64 + lane;
+#pragma unroll
+    for (int n = 0; n < NBT; n++) bp[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
+    i32x4 acc[MB][NBT];
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < NBT; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+    int k = kb;
+    for (; k + U <= ke; k += U) {
+        i32x4 a[U][MB], b[U][NBT];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int m = 0; m < MB; m++) a[u][m] = ap[m][(size_t)(k + u) * 64];
+#pragma unroll
+            for (int n = 0; n < NBT; n++) b[u][n] = __builtin_nontemporal_load(bp[n] + (size_t)(k + u) * BS);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int m = 0; m < MB; m++)
+#pragma unroll
+                for (int n = 0; n < NBT; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][m], b[u][n], acc[m][n], 0, 0, 0);
+    }
+    for (; k < ke; k++) {
+        i32x4 a[MB], b[NBT];
+#pragma unroll
+        for (int m = 0; m < MB; m++) a[m] = ap[m][(size_t)k * 64];
+#pragma unroll
+        for (int n = 0; n < NBT; n++) b[n] = __builtin_nontemporal_load(bp[n] + (size_t)k * BS);
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int n = 0; n < NBT; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], b[n], acc[m][n], 0, 0, 0);
+    }
 #pragma unroll
     for (int m = 0; m < MB; m++) {
-        const int c = 4 * (mt0 + m) + (lane >> 4);
-        const uint32_t yc = m + mt0 < MT ? addmod(mulmod(red64(ysum[c]), k128e), ycorr0) : 0;
+        if (mt0 + m >= g.MT) continue;
+        if (PACK) {
 #pragma unroll
-        for (int n = 0; n < NB; n++) {
-            if (mt0 + m >= MT || ct0 + n >= CT) continue;
-            const int col = 16 * (ct0 + n) + (lane & 15);
-            int64_t V[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int n = 0; n < NBT; n += 4)
+                if (bt0 + n < g.NT) pack_store(g, mt0 + m, (bt0 + n) >> 2, &acc[m][n]);
+        } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                V[j + 0] += acc[m][n][j].x;
-                V[j + 1] += acc[m][n][j].y;
-                V[j + 2] += acc[m][n][j].z;
-                V[j + 3] += acc[m][n][j].w;
+            for (int n = 0; n < NBT; n++)
+                if (bt0 + n < g.NT) scan_store<ATOMIC>(g, mt0 + m, bt0 + n, acc[m][n]);
+        }
+    }
+}
+
+// LDS-staged contraction for many queries (matrix-core bound).  Workgroup = WM x WN waves,
+// wave tile TM x TN blocks, workgroup tile BM = WM*TM A tiles x BN = WN*TN B tiles.  Each
+// k-step the workgroup copies its BM + BN operand blocks (1 KiB each) into one half of a
+// double-buffered LDS area while the matrix cores work on the other half; one barrier per
+// k-step.
+template <int TM, int TN, int WM, int WN, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(256, 1) void lds_kernel(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                                     GemmArgs g) {
+    static_assert(WM * WN == 4, "four waves");
+    constexpr int BM = WM * TM, BN = WN * TN, NBLK = BM + BN;
+    static_assert(NBLK % 4 == 0, "blocks split evenly over the waves");
+    static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    constexpr int PW = NBLK / 4;
+    constexpr int BS = b_kstride<PACK>();
+    __shared__ i32x4 lds[2][NBLK][64];
+    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave / WN, wn = wave % WN;
+    const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    // this wave copies blocks wave, wave + 4, ... of the workgroup's NBLK
+    const i32x4 *src[PW];
+    int kst[PW];
+#pragma unroll
+    for (int p = 0; p < PW; p++) {
+        const int blk = wave + 4 * p;
+        if (blk < BM) {
+            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            kst[p] = 64;
+        } else {
+            src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
+            kst[p] = BS;
+        }
+    }
+    i32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+    {
+        i32x4 pre[PW];
+#pragma unroll
+        for (int p = 0; p < PW; p++) pre[p] = src[p][(size_t)kb * kst[p]];
+#pragma unroll
+        for (int p = 0; p < PW; p++) lds[0][wave + 4 * p][lane] = pre[p];
+    }
+    __syncthreads();
+    for (int k = kb; k < ke; k++) {
+        const int buf = (k - kb) & 1;
+        const bool more = k + 1 < ke;
+        i32x4 pre[PW];
+        if (more) {
+#pragma unroll
+            for (int p = 0; p < PW; p++) pre[p] = src[p][(size_t)(k + 1) * kst[p]];
+        }
+        i32x4 b[TN];
+#pragma unroll
+        for (int n = 0; n < TN; n++) b[n] = lds[buf][BM + wn * TN + n][lane];
+#pragma unroll
+        for (int m = 0; m < TM; m++) {
+            const i32x4 a = lds[buf][wm * TM + m][lane];
+#pragma unroll
+            for (int n = 0; n < TN; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[m][n], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int p = 0; p < PW; p++) lds[buf ^ 1][wave + 4 * p][lane] = pre[p];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int rt = mt0 + wm * TM + m;
+        if (rt >= g.MT) continue;
+        if (PACK) {
+#pragma unroll
+            for (int n = 0; n < TN; n += 4) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
             }
-            const int64_t t = V[0] + (V[1] << 8) + (V[2] << 16) + (V[3] << 24) + V[4] * w4 + V[5] * w5 + V[6] * w6;
-            beta[(size_t)c * C + col] = addmod(addmod(red_i64(t), ucorr[col]), yc);
+        } else {
+#pragma unroll
+            for (int n = 0; n < TN; n++) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+            }
+        }
+    }
+}
+
+// vmcnt(n) with the other counters left alone (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] |
+// lgkmcnt[11:8] | vmcnt[5:4] << 14).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier without the memory fence of __syncthreads(): the fence would wait for
+// every outstanding global load, including the prefetches of later stages.  The empty asm
+// statements keep the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void stage_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Multi-stage contraction for many queries (matrix-core bound).  Like lds_kernel, but the
+// operand blocks travel from memory straight into LDS (global_load_lds_dwordx4: lane l's
+// 16 bytes land at 16 * l of the destination, which is exactly the block layout) through a
+// ring of S stages, so S - 1 k-steps of loads are in flight while the matrix cores work
+// and no VGPRs hold prefetched data.  Needs S * (BM + BN) KiB of dynamic LDS.
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(256, 1) void dma_kernel(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                                     GemmArgs g) {
+    static_assert(WM * WN == 4, "four waves");
+    static_assert(S >= 2 && S <= 4, "stages");
+    constexpr int BM = WM * TM, BN = WN * TN, NBLK = BM + BN;
+    static_assert(NBLK % 4 == 0, "blocks split evenly over the waves");
+    static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    constexpr int PW = NBLK / 4;
+    constexpr int BS = b_kstride<PACK>();
+    extern __shared__ i32x4 ring[];  // [S][NBLK][64]
+    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave / WN, wn = wave % WN;
+    const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    const i32x4 *src[PW];
+    int kst[PW];
+#pragma unroll
+    for (int p = 0; p < PW; p++) {
+        const int blk = wave + 4 * p;
+        if (blk < BM) {
+            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            kst[p] = 64;
+        } else {
+            src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
+            kst[p] = BS;
+        }
+    }
+    auto issue = [&](int k) {
+        i32x4 *dst = ring + (size_t)(((k - kb) % S) * NBLK) * 64;
+#pragma unroll
+        for (int p = 0; p < PW; p++)
+            __builtin_amdgcn_global_load_lds((const void *)(src[p] + (size_t)k * kst[p]),
+                                             (lds_void *)(dst + (wave + 4 * p) * 64), 16, 0, 0);
+    };
+    i32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S - 1; s++)
+        if (kb + s < ke) issue(kb + s);
+    for (int k = kb; k < ke; k++) {
+        // this wave's loads of step k are done once at most (later steps issued) x PW remain
+        const int later = min(S - 2, ke - 1 - k);
+        if (later >= 2) wait_vmcnt<2 * PW>();
+        else if (later == 1) wait_vmcnt<PW>();
+        else wait_vmcnt<0>();
+        stage_barrier();
+        if (k + S - 1 < ke) issue(k + S - 1);
+        const i32x4 *cur = ring + (size_t)(((k - kb) % S) * NBLK) * 64;
+        i32x4 b[TN];
+#pragma unroll
+        for (int n = 0; n < TN; n++) b[n] = cur[(BM + wn * TN + n) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < TM; m++) {
+            const i32x4 a = cur[(wm * TM + m) * 64 + lane];
+#pragma unroll
+            for (int n = 0; n < TN; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[m][n], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int rt = mt0 + wm * TM + m;
+        if (rt >= g.MT) continue;
+        if (PACK) {
+#pragma unroll
+            for (int n = 0; n < TN; n += 4) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < TN; n++) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+            }
         }
     }
 }
@@ -415,6 +607,22 @@ struct TailArgs {
     uint32_t q1;
 };
 
+// trunc(x / Q) for |x| < 2^53, exactly: a double-precision estimate (off by at most one
+// near an integer quotient) corrected with the exact integer remainder.
+__device__ __forceinline__ int64_t div_q_trunc(int64_t x) {
+    const int64_t Q = (int64_t)kQ;
+    int64_t t = (int64_t)((double)x * (1.0 / (double)kQ));
+    const int64_t r = x - t * Q;
+    if (x >= 0) {
+        if (r < 0) t--;
+        else if (r >= Q) t++;
+    } else {
+        if (r > 0) t++;
+        else if (r <= -Q) t--;
+    }
+    return t;
+}
+
 // Row-1 rescale Q -> q1 (src/amx_kernels.cpp:522-529): centre, round half away from zero,
 // wrap into [0, q1).  Exact integer form of the reference's double-precision division.
 __device__ __forceinline__ uint32_t rescale_q1(uint32_t v, uint32_t q1) {
@@ -422,66 +630,107 @@ __device__ __forceinline__ uint32_t rescale_q1(uint32_t v, uint32_t q1) {
     const bool neg = (int64_t)v >= half;
     const int64_t c = neg ? (int64_t)v - (int64_t)kQ : (int64_t)v;
     const int64_t x = c * (int64_t)q1 + (neg ? -half : half);
-    int64_t t = x / (int64_t)kQ;
+    int64_t t = div_q_trunc(x);
     if (t < 0) t += q1;
     if (t >= (int64_t)q1) t -= q1;
     return (uint32_t)t;
 }
 
+__device__ __forceinline__ void gs_butterfly(uint32_t &x, uint32_t &y, uint32_t w) {
+    uint64_t s = (uint64_t)x + y;
+    if (s >= kQ) s -= kQ;
+    s = (s & 1) ? (s + kQ) >> 1 : s >> 1;  // (x + y) / 2 mod Q
+    const uint32_t d = red_once((uint64_t)x + kQ - y);
+    x = (uint32_t)s;
+    y = mulmod(d, w);
+}
+
+__device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *(const uint4 *)p; }
+
 // One workgroup per (output o, query c): pointwise key product, Gentleman-Sande inverse
-// NTT in LDS (src/amx_kernels.cpp:453-513), row-1 switch and LSB-first bit packing ORed
-// onto the precomputed prefix (src/amx_kernels.cpp:515-537, 604-607).
+// NTT (src/amx_kernels.cpp:453-513), row-1 switch and LSB-first bit packing ORed onto the
+// precomputed prefix (src/amx_kernels.cpp:515-537, 604-607).
+//
+// The transform runs in radix-8 phases: in phase p a thread holds a group of 8 elements
+// base + 8^p * m (m < 8) in registers and applies the (up to) three stages t = 8^p, 2*8^p,
+// 4*8^p that stay inside the group; groups are exchanged through LDS between phases, so
+// n = 2048 needs three barriers instead of eleven.  Phase 0 takes its group straight from
+// the pointwise product and the last phase feeds the switch directly.
 __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     extern __shared__ uint32_t lds[];
     const int o = blockIdx.x, c = blockIdx.y;
     if (p.status[c] != 0) return;
     uint32_t *a = lds;
     uint32_t *part = lds + p.n;
-    const int n = p.n;
+    const int n = p.n, logn = p.logn, groups = n >> 3;
     const size_t col0 = (size_t)o * n;
+    const uint32_t *pre = (const uint32_t *)(p.prefix + (size_t)o * p.part_words * 4);
+    for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) part[w] = pre[w];
     const uint32_t *zk = (const uint32_t *)(p.slots + (size_t)c * p.stride + 16) + p.kp;
     const uint32_t *th = p.that + (size_t)o * p.kp;
     const uint32_t *bc = p.beta + (size_t)c * p.C + col0;
-    for (int z = threadIdx.x; z < n; z += blockDim.x) {
-        uint64_t s = bc[z];
-        for (int k = 0; k < p.tg; k++) s += mulmod(zk[k * n + z], th[k * n + z]);
-        a[z] = red64(s);
-    }
-    const uint32_t *pre = (const uint32_t *)(p.prefix + (size_t)o * p.part_words * 4);
-    for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) part[w] = pre[w];
-    __syncthreads();
-    for (int lt = 0; lt < p.logn; lt++) {
-        const int t = 1 << lt, h = n >> (lt + 1);
-        for (int b = threadIdx.x; b < n / 2; b += blockDim.x) {
-            const int i = b >> lt, j = b & (t - 1);
-            const int xi = 2 * t * i + j;
-            const uint32_t x = a[xi], y = a[xi + t];
-            uint64_t s = (uint64_t)x + y;
-            if (s >= kQ) s -= kQ;
-            s = (s & 1) ? (s + kQ) >> 1 : s >> 1;
-            const uint32_t d = red_once((uint64_t)x + kQ - y);
-            a[xi] = (uint32_t)s;
-            a[xi + t] = mulmod(d, p.wtab[h + i]);
+    const uint32_t *accc = p.acc + (size_t)c * p.C + col0;
+    const uint32_t qs = p.qsum[c] << 7;
+    const int nph = (logn + 2) / 3;
+    for (int ph = 0; ph < nph; ph++) {
+        const int ls = 3 * ph, sp = 1 << ls;
+        for (int g = threadIdx.x; g < groups; g += blockDim.x) {
+            const int base = ((g >> ls) << (ls + 3)) | (g & (sp - 1));
+            uint32_t v[8];
+            if (ph == 0) {
+                // pointwise: a = beta + sum_k z * that (mod Q), elements 8g .. 8g + 7
+                uint64_t s8[8];
+                const uint4 b0 = ld4(bc + base), b1 = ld4(bc + base + 4);
+                s8[0] = b0.x; s8[1] = b0.y; s8[2] = b0.z; s8[3] = b0.w;
+                s8[4] = b1.x; s8[5] = b1.y; s8[6] = b1.z; s8[7] = b1.w;
+                for (int k = 0; k < p.tg; k++) {
+                    const uint4 z0 = ld4(zk + k * n + base), z1 = ld4(zk + k * n + base + 4);
+                    const uint4 t0 = ld4(th + k * n + base), t1 = ld4(th + k * n + base + 4);
+                    s8[0] += mulmod(z0.x, t0.x); s8[1] += mulmod(z0.y, t0.y);
+                    s8[2] += mulmod(z0.z, t0.z); s8[3] += mulmod(z0.w, t0.w);
+                    s8[4] += mulmod(z1.x, t1.x); s8[5] += mulmod(z1.y, t1.y);
+                    s8[6] += mulmod(z1.z, t1.z); s8[7] += mulmod(z1.w, t1.w);
+                }
+#pragma unroll
+                for (int m = 0; m < 8; m++) v[m] = red64(s8[m]);
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; m++) v[m] = a[base + (m << ls)];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int lt = ls + q;
+                if (lt >= logn) break;
+                const int h = n >> (lt + 1);
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    if (m & (1 << q)) continue;
+                    const int xi = base + (m << ls);
+                    gs_butterfly(v[m], v[m | (1 << q)], p.wtab[h + (xi >> (lt + 1))]);
+                }
+            }
+            if (ph + 1 < nph) {
+#pragma unroll
+                for (int m = 0; m < 8; m++) a[base + (m << ls)] = v[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const int z = base + (m << ls);
+                    const uint32_t acc = accc[z] + qs;
+                    const uint32_t inter = (uint32_t)(((uint64_t)acc * kQ + (1ull << 31)) >> 32);
+                    const uint32_t r = rescale_q1(addmod(inter, v[m]), p.q1);
+                    const unsigned long long bit = (unsigned long long)p.row1_off + (unsigned long long)z * p.q1_bits;
+                    const unsigned long long byte0 = bit >> 3;
+                    // the reference ORs (r << bit%8) into the four bytes from byte0 on, inside the part
+                    const uint64_t vv = ((uint64_t)r << (bit & 7)) << ((byte0 & 3) * 8);  // < 2^52
+                    const unsigned long long wi = byte0 >> 2;
+                    if (wi < (unsigned long long)p.part_words) atomicOr(&part[wi], (uint32_t)vv);
+                    if (wi + 1 < (unsigned long long)p.part_words && (vv >> 32)) atomicOr(&part[wi + 1], (uint32_t)(vv >> 32));
+                }
+            }
         }
         __syncthreads();
     }
-    const uint32_t *accc = p.acc + (size_t)c * p.C + col0;
-    const uint32_t qs = p.qsum[c] << 7;
-    for (int z = threadIdx.x; z < n; z += blockDim.x) {
-        const uint32_t acc = accc[z] + qs;
-        const uint32_t inter = (uint32_t)(((uint64_t)acc * kQ + (1ull << 31)) >> 32);
-        const uint32_t r = rescale_q1(addmod(inter, a[z]), p.q1);
-        const unsigned long long bit = (unsigned long long)p.row1_off + (unsigned long long)z * p.q1_bits;
-        const unsigned long long byte0 = bit >> 3;
-        // the reference ORs (r << bit%8) into the four bytes from byte0 on, inside the part
-        const uint64_t v = (uint64_t)r << (bit & 7);
-        const unsigned long long wi = byte0 >> 2;
-        const int sh = (int)(byte0 & 3) * 8;
-        const uint64_t vv = v << sh;  // r < 2^21, so vv < 2^52
-        if (wi < (unsigned long long)p.part_words) atomicOr(&part[wi], (uint32_t)vv);
-        if (wi + 1 < (unsigned long long)p.part_words && (vv >> 32)) atomicOr(&part[wi + 1], (uint32_t)(vv >> 32));
-    }
-    __syncthreads();
     uint8_t *dst = p.out + (size_t)c * p.out_stride + (size_t)o * p.part_len;
     if ((((uintptr_t)dst) & 3) == 0 && (p.part_len & 3) == 0) {
         uint32_t *d32 = (uint32_t *)dst;

@@ -117,7 +117,13 @@ def main() -> int:
     slots = torch.from_numpy(host).to("cuda")
     status = torch.zeros(B, dtype=torch.int32, device="cuda")
     out = torch.empty((B, rl), dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+    # a stream of our own (default priority): the engine launches on it and the timing
+    # events below are recorded on it
+    work = torch.cuda.Stream()
+    torch.cuda.set_stream(work)
+    stream = work.cuda_stream
+    assert stream, "expected a non-null stream handle"
 
     def step():
         rc = eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(), stream)

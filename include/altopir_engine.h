@@ -101,6 +101,15 @@ int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t
  * `count` values and returns how many were written (0 if no batch ran yet). */
 int alto_last_timings(void *e, double *ms, int count);
 
+/* Kernel variants of the two contractions (scan and packing) and the per-engine tuning
+ * table that picks one per batch-size bucket (buckets: ceil(log2(ceil(B / 4))))).
+ * which: 0 = scan, 1 = packing; idx -1 restores the built-in choice.  Every variant
+ * computes the same bytes; they differ only in tile shapes and operand staging. */
+int alto_variant_count(void);
+const char *alto_variant_name(int idx);
+int alto_tune_set(void *e, int which, size_t B, int idx);
+int alto_tune_get(const void *e, int which, size_t B);
+
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */
 size_t alto_engine_device_bytes(const void *e);

@@ -174,3 +174,26 @@ def test_repeated_batches_of_different_sizes_reuse_the_engine(hip_device, small_
         oc.oracle_answer(inp, reqs, out_o, np.full(B, -1, np.int32))
         assert np.array_equal(out, out_o), f"batch {B}"
     eng.close()
+
+
+@pytest.mark.parametrize("B", [1, 6, 13, 72])
+def test_every_kernel_variant_gives_the_same_bytes(hip_device, B):
+    from altopir_public_amd import _lib, autotune
+
+    inp = _block_inputs()
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=500 + B)
+    rl = inp.response_len
+    out_o = np.zeros((B, rl), np.uint8)
+    oc.oracle_answer(inp, reqs, out_o, np.full(B, -1, np.int32))
+    eng = Engine.from_inputs(inp)
+    lib = _lib.load()
+    names = autotune.variant_names()
+    for i, name in enumerate(names):
+        for which in (0, 1):
+            assert lib.alto_tune_set(eng.ptr, which, B, i) == 0
+        out = np.zeros((B, rl), np.uint8)
+        st = np.full(B, -1, np.int32)
+        assert eng.answer(reqs, out, st)[0] == B
+        bad = [c for c in range(B) if not np.array_equal(out[c], out_o[c])]
+        assert not bad, f"variant {name}: responses {bad[:8]} differ"
+    eng.close()
