@@ -15,7 +15,12 @@ SYMBOLS = (
     "sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build",
     "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
     "alto_answer_device", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
-    "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get",
+    "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
+    "alto_engine_new_device", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
+    "alto_answer_gather", "alto_context_timings",
+    # serving loop (bound in serve.py)
+    "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
+    "alto_server_get_stats", "alto_server_error", "alto_server_free",
 )
 
 _lib = None
@@ -70,6 +75,23 @@ def load() -> C.CDLL:
     lib.alto_tune_set.argtypes = [P, C.c_int, SZ, C.c_int]
     lib.alto_tune_get.restype = C.c_int
     lib.alto_tune_get.argtypes = [P, C.c_int, SZ]
+    lib.alto_engine_new_device.restype = P
+    lib.alto_engine_new_device.argtypes = [P, SZ, SZ, SZ, P, SZ, SZ, P, SZ, SZ, C.c_uint64, C.c_int, P, P,
+                                           C.c_uint64, C.c_int]
+    lib.alto_context_new.restype = P
+    lib.alto_context_new.argtypes = [P]
+    lib.alto_context_free.restype = None
+    lib.alto_context_free.argtypes = [P]
+    lib.alto_engine_context.restype = P
+    lib.alto_engine_context.argtypes = [P]
+    lib.alto_context_reserve.restype = C.c_int
+    lib.alto_context_reserve.argtypes = [P, SZ]
+    lib.alto_answer_gather.restype = C.c_int
+    lib.alto_answer_gather.argtypes = [P, P, SZ, P, SZ, P, SZ, P, P, P]
+    lib.alto_context_timings.restype = C.c_int
+    lib.alto_context_timings.argtypes = [P, P, C.c_int]
+    lib.alto_set_graphs.restype = C.c_int
+    lib.alto_set_graphs.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 

@@ -39,7 +39,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / "alto_engine.hip")]
+    srcs = [str(CSRC / "alto_engine.hip"), str(CSRC / "alto_serve.hip")]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", "-o", str(tmp), *srcs]
     if verbose:

@@ -87,18 +87,22 @@ void launch_lds(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
                            (const i32x4 *)b, g);
 }
 
+// Allows the dynamic LDS of a dma_kernel instantiation (done once per engine creation,
+// outside any stream capture).
+template <int TM, int TN, int WM, int WN, int S, bool PACK>
+void prepare_dma() {
+    constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+void prepare_none() {}
+
 template <int TM, int TN, int WM, int WN, int S, bool PACK>
 void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     constexpr int NBLK = WM * TM + WN * TN;
     constexpr size_t lds = (size_t)S * NBLK * 1024;
-    static bool attr_set = false;  // per instantiation; the attribute is per function
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        attr_set = true;
-    }
     g.gx = ceil_div(g.NT, WN * TN);
     g.gy = ceil_div(g.MT, WM * TM);
     if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
@@ -116,20 +120,24 @@ void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
 // of A tiles (ragged tiles are clamped and masked); they differ in tile shapes and in how
 // operands are staged, which the tuner chooses per batch size.
 typedef void (*LaunchFn)(GemmArgs, const int8_t *, const int8_t *, hipStream_t);
+typedef void (*PrepareFn)();
 struct Variant {
     const char *name;
     LaunchFn scan, pack;
+    PrepareFn prepare_scan, prepare_pack;
 };
-#define ALTO_VARIANT(name, ...) {name, __VA_ARGS__<false>, __VA_ARGS__<true>}
 const Variant kVariants[] = {
-    {"stream_m1_n4_u4", launch_stream<1, 4, 4, false>, launch_stream<1, 4, 4, true>},
-    {"stream_m2_n4_u4", launch_stream<2, 4, 4, false>, launch_stream<2, 4, 4, true>},
-    {"stream_m4_n4_u2", launch_stream<4, 4, 2, false>, launch_stream<4, 4, 2, true>},
-    {"stream_m8_n4_u2", launch_stream<8, 4, 2, false>, launch_stream<8, 4, 2, true>},
-    {"lds_m8_n8", launch_lds<8, 8, 2, 2, false>, launch_lds<8, 8, 2, 2, true>},
-    {"dma_m4_n8_s4", launch_dma<4, 8, 2, 2, 4, false>, launch_dma<4, 8, 2, 2, 4, true>},
-    {"dma_m8_n8_s3", launch_dma<8, 8, 2, 2, 3, false>, launch_dma<8, 8, 2, 2, 3, true>},
-    {"dma_m8_n8_s4", launch_dma<8, 8, 2, 2, 4, false>, launch_dma<8, 8, 2, 2, 4, true>},
+    {"stream_m1_n4_u4", launch_stream<1, 4, 4, false>, launch_stream<1, 4, 4, true>, prepare_none, prepare_none},
+    {"stream_m2_n4_u4", launch_stream<2, 4, 4, false>, launch_stream<2, 4, 4, true>, prepare_none, prepare_none},
+    {"stream_m4_n4_u2", launch_stream<4, 4, 2, false>, launch_stream<4, 4, 2, true>, prepare_none, prepare_none},
+    {"stream_m8_n4_u2", launch_stream<8, 4, 2, false>, launch_stream<8, 4, 2, true>, prepare_none, prepare_none},
+    {"lds_m8_n8", launch_lds<8, 8, 2, 2, false>, launch_lds<8, 8, 2, 2, true>, prepare_none, prepare_none},
+    {"dma_m4_n8_s4", launch_dma<4, 8, 2, 2, 4, false>, launch_dma<4, 8, 2, 2, 4, true>,
+     prepare_dma<4, 8, 2, 2, 4, false>, prepare_dma<4, 8, 2, 2, 4, true>},
+    {"dma_m8_n8_s3", launch_dma<8, 8, 2, 2, 3, false>, launch_dma<8, 8, 2, 2, 3, true>,
+     prepare_dma<8, 8, 2, 2, 3, false>, prepare_dma<8, 8, 2, 2, 3, true>},
+    {"dma_m8_n8_s4", launch_dma<8, 8, 2, 2, 4, false>, launch_dma<8, 8, 2, 2, 4, true>,
+     prepare_dma<8, 8, 2, 2, 4, false>, prepare_dma<8, 8, 2, 2, 4, true>},
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
@@ -150,27 +158,9 @@ int bucket_of(int MT) {
     return b;
 }
 
-// ------------------------------------------------------------------ workspace --
+// -------------------------------------------------------------------- engine --
 
-struct Workspace {
-    size_t cap = 0;              // queries (padded to 4)
-    size_t slot_stride = 0;
-    uint8_t *slots = nullptr;    // host-API request arena on the device
-    uint8_t *out = nullptr;      // host-API response arena on the device
-    int8_t *qa = nullptr;        // scan limbs, cap * 4 * rows bytes
-    int8_t *ya = nullptr;        // packing limbs, cap * 4 * KP bytes
-    uint32_t *acc = nullptr;     // cap * C
-    uint32_t *beta = nullptr;    // cap * C
-    uint32_t *qsum = nullptr;    // cap
-    unsigned long long *ysum = nullptr;
-    int *status = nullptr;       // cap
-    void release() {
-        for (void *p : {(void *)slots, (void *)out, (void *)qa, (void *)ya, (void *)acc, (void *)beta, (void *)qsum,
-                        (void *)ysum, (void *)status})
-            if (p) (void)hipFree(p);
-        *this = Workspace();
-    }
-};
+struct Context;
 
 struct Engine {
     int device = 0;
@@ -188,131 +178,199 @@ struct Engine {
     uint8_t *prefix = nullptr;   // outputs x part_words * 4
     size_t device_bytes = 0;
     uint32_t k128e = 0, ycorr0 = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[7] = {};  // start, validate, scan prep, scan, pack prep, pack, tail
-    bool timed_valid = false;
-    Workspace ws;
-    std::vector<int> hstatus;
-    int tuned[2][kBuckets];  // [scan, pack][bucket] -> variant, -1 = built-in policy
+    int tuned[2][kBuckets];      // [scan, pack][bucket] -> variant, -1 = built-in policy
+    bool use_graphs = true;
+    Context *ctx0 = nullptr;     // the context of alto_answer / alto_answer_device
 
     Engine() {
         for (auto &t : tuned)
             for (int &v : t) v = -1;
     }
+    ~Engine();
     int variant_for(int which, int MT) const {
         const int v = tuned[which][bucket_of(MT)];
         return v >= 0 ? v : default_variant(MT);
     }
-
-    ~Engine() {
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        ws.release();
-        for (void *p : {(void *)db, (void *)u, (void *)ucorr, (void *)that, (void *)wtab, (void *)prefix})
-            if (p) (void)hipFree(p);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
     size_t request_len() const { return 9 + 4 * (2 * kp + rows); }
     size_t response_len() const { return outputs * part_len; }
+};
+
+// ------------------------------------------------------------------ contexts --
+
+// One batch in flight: its workspace, stream, timing events and captured graphs.  Batches
+// on different contexts of one engine may run concurrently; one context runs one batch at
+// a time.
+struct Context {
+    Engine *e = nullptr;
+    size_t cap = 0;              // queries (multiple of 4)
+    int8_t *qa = nullptr;        // scan limbs, cap * 4 * rows bytes
+    int8_t *ya = nullptr;        // packing limbs, cap * 4 * KP bytes
+    uint32_t *acc = nullptr;     // cap * C
+    uint32_t *beta = nullptr;    // cap * C
+    uint32_t *qsum = nullptr;    // cap
+    unsigned long long *ysum = nullptr;
+    // arenas of the synchronous entry point (allocated on first use)
+    size_t slot_stride = 0;
+    uint8_t *slots = nullptr, *out = nullptr;
+    int *status = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[7] = {};  // start, validate, scan prep, scan, pack prep, pack, tail
+    bool timed_valid = false;
+    std::vector<int> hstatus;
+    // captured batches, replayed when the same batch (pointers, size, stream, kernel
+    // variants) comes again
+    struct GraphEntry {
+        FrameSet fs;
+        size_t B;
+        uint8_t *out;
+        size_t out_stride;
+        const int *oidx;
+        int *status;
+        hipStream_t st;
+        int vs, vp;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t tick = 0;
+    static constexpr size_t kMaxGraphs = 64;
+
+    bool init(Engine *eng) {
+        e = eng;
+        bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &x : ev) ok = ok && HIP_OK(hipEventCreate(&x));
+        return ok;
+    }
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+    void release_buffers() {
+        for (void *p : {(void *)qa, (void *)ya, (void *)acc, (void *)beta, (void *)qsum, (void *)ysum, (void *)slots,
+                        (void *)out, (void *)status})
+            if (p) (void)hipFree(p);
+        qa = ya = nullptr;
+        acc = beta = qsum = nullptr;
+        ysum = nullptr;
+        slots = out = nullptr;
+        status = nullptr;
+        cap = 0;
+    }
+    ~Context() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        release_buffers();
+        for (auto &x : ev)
+            if (x) (void)hipEventDestroy(x);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 
     // Grows the per-batch buffers to hold `B` queries; host_arenas also sizes the request
     // and response arenas of the synchronous entry point.
     bool reserve(size_t B, bool host_arenas) {
         const size_t Bp = round_up(std::max<size_t>(B, 1), 4);
-        const bool grow = Bp > ws.cap;
-        const bool add_host = host_arenas && !ws.slots;
+        const bool grow = Bp > cap;
+        const bool add_host = host_arenas && !slots;
         if (!grow && !add_host) return true;
         (void)hipStreamSynchronize(stream);
-        const size_t cap = grow ? round_up(std::max(Bp, ws.cap + ws.cap / 2), 4) : ws.cap;
-        const bool with_host = host_arenas || ws.slots;
-        ws.release();
-        ws.cap = cap;
-        ws.slot_stride = round_up(ALTO_FRAME_OFFSET + request_len(), 256);
-        bool ok = HIP_OK(hipMalloc(&ws.qa, cap * 4 * rows)) && HIP_OK(hipMalloc(&ws.ya, cap * 4 * kp)) &&
-                  HIP_OK(hipMalloc(&ws.acc, cap * cols * 4)) && HIP_OK(hipMalloc(&ws.beta, cap * cols * 4)) &&
-                  HIP_OK(hipMalloc(&ws.qsum, cap * 4)) && HIP_OK(hipMalloc(&ws.ysum, cap * 8)) &&
-                  HIP_OK(hipMalloc(&ws.status, cap * 4));
+        (void)hipDeviceSynchronize();  // a caller stream may still be using the buffers
+        drop_graphs();                 // they hold the old workspace addresses
+        const size_t ncap = grow ? round_up(std::max(Bp, cap + cap / 2), 4) : cap;
+        const bool with_host = host_arenas || slots;
+        release_buffers();
+        cap = ncap;
+        slot_stride = round_up(ALTO_FRAME_OFFSET + e->request_len(), 256);
+        bool ok = HIP_OK(hipMalloc(&qa, cap * 4 * e->rows)) && HIP_OK(hipMalloc(&ya, cap * 4 * e->kp)) &&
+                  HIP_OK(hipMalloc(&acc, cap * e->cols * 4)) && HIP_OK(hipMalloc(&beta, cap * e->cols * 4)) &&
+                  HIP_OK(hipMalloc(&qsum, cap * 4)) && HIP_OK(hipMalloc(&ysum, cap * 8));
         if (ok && with_host)
-            ok = HIP_OK(hipMalloc(&ws.slots, cap * ws.slot_stride)) && HIP_OK(hipMalloc(&ws.out, cap * response_len()));
-        if (!ok) ws.release();
+            ok = HIP_OK(hipMalloc(&slots, cap * slot_stride)) && HIP_OK(hipMalloc(&out, cap * e->response_len())) &&
+                 HIP_OK(hipMalloc(&status, cap * 4));
+        if (!ok) release_buffers();
         return ok;
     }
 
-    int enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out, size_t out_stride, int *status,
-                hipStream_t st, bool timed, bool record_start = true);
+    int enqueue(FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx, int *status, hipStream_t st,
+                bool timed, bool record_start = true);
+    int answer_device(FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx, int *status,
+                      hipStream_t st);
 };
+
+Engine::~Engine() {
+    (void)hipSetDevice(device);
+    delete ctx0;
+    for (void *p : {(void *)db, (void *)u, (void *)ucorr, (void *)that, (void *)wtab, (void *)prefix})
+        if (p) (void)hipFree(p);
+}
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-int Engine::enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out, size_t out_stride, int *status,
-                    hipStream_t st, bool timed, bool record_start) {
+int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
+                     hipStream_t st, bool timed, bool record_start) {
+    const Engine &E = *e;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
     if (timed && record_start) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, slots, stride, status, (uint32_t)kp,
-                       ws.qsum, ws.ysum);
+    hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, st_dev, (uint32_t)E.kp, qsum, ysum);
     if (timed) (void)hipEventRecord(ev[1], st);
-    // scan: query limbs, then the contraction into ws.acc
-    hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0, st,
-                       slots, stride, (size_t)(2 * kp), (const int *)status, (int)B, KC, ws.qa, ws.qsum, ws.ysum);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, ws.acc, Bp * cols, 0u);
+    // scan: query limbs, then the contraction into acc
+    hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((E.KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
+                       st, fs, (size_t)(2 * E.kp), (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
     if (timed) (void)hipEventRecord(ev[2], st);
     {
         GemmArgs g{};
-        g.KC = (int)KC;
+        g.KC = (int)E.KC;
         g.MT = (int)MT;
-        g.NT = (int)CT;
-        g.C = (int)cols;
-        g.out = ws.acc;
-        kVariants[variant_for(0, (int)MT)].scan(g, ws.qa, db, st);
+        g.NT = (int)E.CT;
+        g.C = (int)E.cols;
+        g.out = acc;
+        kVariants[E.variant_for(0, (int)MT)].scan(g, qa, E.db, st);
     }
     if (timed) (void)hipEventRecord(ev[3], st);
-    // packing: key limbs, then the contraction into ws.beta
-    hipLaunchKernelGGL((limb_prep_kernel<false>), dim3((unsigned)((KCp * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
-                       st, slots, stride, (size_t)0, (const int *)status, (int)B, KCp, ws.ya, ws.qsum, ws.ysum);
+    // packing: key limbs, then the contraction into beta
+    hipLaunchKernelGGL((limb_prep_kernel<false>), dim3((unsigned)((E.KCp * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
+                       st, fs, (size_t)0, (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
     if (timed) (void)hipEventRecord(ev[4], st);
     {
         GemmArgs g{};
-        g.KC = (int)KCp;
+        g.KC = (int)E.KCp;
         g.MT = (int)MT;
-        g.NT = (int)(4 * CT);
-        g.C = (int)cols;
-        g.out = ws.beta;
-        g.ucorr = ucorr;
-        g.ysum = ws.ysum;
-        g.k128e = k128e;
-        g.ycorr0 = ycorr0;
-        kVariants[variant_for(1, (int)MT)].pack(g, ws.ya, u, st);
+        g.NT = (int)(4 * E.CT);
+        g.C = (int)E.cols;
+        g.out = beta;
+        g.ucorr = E.ucorr;
+        g.ysum = ysum;
+        g.k128e = E.k128e;
+        g.ycorr0 = E.ycorr0;
+        kVariants[E.variant_for(1, (int)MT)].pack(g, ya, E.u, st);
     }
     if (timed) (void)hipEventRecord(ev[5], st);
     TailArgs ta;
-    ta.slots = slots;
-    ta.stride = stride;
-    ta.status = status;
-    ta.acc = ws.acc;
-    ta.qsum = ws.qsum;
-    ta.beta = ws.beta;
-    ta.that = that;
-    ta.wtab = wtab;
-    ta.prefix = prefix;
-    ta.out = out;
+    ta.fs = fs;
+    ta.status = st_dev;
+    ta.acc = acc;
+    ta.qsum = qsum;
+    ta.beta = beta;
+    ta.that = E.that;
+    ta.wtab = E.wtab;
+    ta.prefix = E.prefix;
+    ta.out = outp;
     ta.out_stride = out_stride;
-    ta.n = (int)n;
-    ta.logn = logn;
-    ta.tg = (int)tg;
-    ta.C = (int)cols;
-    ta.kp = (int)kp;
-    ta.part_len = (int)part_len;
-    ta.part_words = (int)part_words;
-    ta.row1_off = (long long)row1_off;
-    ta.q1_bits = q1_bits;
-    ta.q1 = (uint32_t)q1;
-    const size_t lds = (n + part_words) * 4;
-    hipLaunchKernelGGL(tail_kernel, dim3((unsigned)outputs, (unsigned)B), dim3(256), lds, st, ta);
+    ta.oidx = oidx;
+    ta.n = (int)E.n;
+    ta.logn = E.logn;
+    ta.tg = (int)E.tg;
+    ta.C = (int)E.cols;
+    ta.kp = (int)E.kp;
+    ta.part_len = (int)E.part_len;
+    ta.part_words = (int)E.part_words;
+    ta.row1_off = (long long)E.row1_off;
+    ta.q1_bits = E.q1_bits;
+    ta.q1 = (uint32_t)E.q1;
+    const size_t lds = (E.n + E.part_words) * 4;
+    hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
     if (timed) (void)hipEventRecord(ev[6], st);
     timed_valid = timed;
     const hipError_t le = hipGetLastError();
@@ -321,6 +379,43 @@ int Engine::enqueue(const uint8_t *slots, size_t stride, size_t B, uint8_t *out,
         return -1;
     }
     return 0;
+}
+
+// The device entry points: eager with stage events when graphs are off, otherwise captured
+// once per distinct batch and replayed.
+int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
+                           hipStream_t st) {
+    if (!e->use_graphs) return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
+    const int MT = (int)(round_up(B, 4) / 4);
+    const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT);
+    timed_valid = false;
+    for (auto &g : graphs)
+        if (g.fs.slots == fs.slots && g.fs.stride == fs.stride && g.fs.sidx == fs.sidx && g.B == B && g.out == outp &&
+            g.out_stride == out_stride && g.oidx == oidx && g.status == st_dev && g.st == st && g.vs == vs &&
+            g.vp == vp) {
+            g.used = ++tick;
+            return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
+        }
+    if (graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                    [](const GraphEntry &a, const GraphEntry &b) { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        graphs.erase(lru);
+    }
+    hipGraph_t graph = nullptr;
+    if (!HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return -1;
+    const int rc = enqueue(fs, B, outp, out_stride, oidx, st_dev, st, false);
+    const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
+    if (rc || !ended) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return -1;
+    }
+    hipGraphExec_t exec = nullptr;
+    const bool inst = HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    if (!inst) return -1;
+    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, exec, ++tick});
+    return HIP_OK(hipGraphLaunch(exec, st)) ? 0 : -1;
 }
 
 bool tail_config_ok(size_t n, uint64_t q1, int q1_bits, size_t row1_off, size_t part_len) {
@@ -409,10 +504,10 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
     });
 }
 
-void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld, size_t rows, size_t cols,
-                      const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len,
-                      size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
-                      const uint64_t *inv_table, uint64_t q, int threads) {
+static void *engine_create(const uint8_t *db_col, bool db_on_device, const uint8_t *db_tiled, size_t ld, size_t rows,
+                           size_t cols, const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,
+                           size_t part_len, size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
+                           const uint64_t *inv_table, uint64_t q, int threads) {
     // configuration checks of src/amx_kernels.cpp:551-552 (the L2-size probe has no GPU
     // meaning), plus empty shapes, which the reference accepts and later aborts on
     if (!db_col || !db_tiled || !ub || !prefix || !that || !inv_table || q != kQ || !n || n > 4096 || n % 16 ||
@@ -441,11 +536,16 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
     e->k128e = (uint32_t)((128ull * kE) % kQ);
     e->ycorr0 = (uint32_t)((kQ - host_mulmod(host_mulmod(e->k128e, e->k128e), e->kp % kQ)) % kQ);
     const size_t lds = (n + e->part_words) * 4;
-    bool ok = HIP_OK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    for (auto &ev : e->ev) ok = ok && HIP_OK(hipEventCreate(&ev));
+    e->ctx0 = new Context;
+    bool ok = e->ctx0->init(e);
+    hipStream_t setup = e->ctx0->stream;
     ok = ok && HIP_OK(hipFuncSetAttribute((const void *)tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)std::min<size_t>(lds, 160 * 1024)));
     if (e->tail_ok && lds > 160 * 1024) e->tail_ok = false;
+    for (const auto &v : kVariants) {
+        v.prepare_scan();
+        v.prepare_pack();
+    }
     // resident state
     const size_t dbb = rows * cols, ubb = 4 * e->kp * cols;
     ok = ok && HIP_OK(hipMalloc(&e->db, dbb)) && HIP_OK(hipMalloc(&e->u, ubb)) &&
@@ -459,23 +559,24 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
     ok = ok && HIP_OK(hipMalloc(&stage, stage_bytes));
     for (size_t c0 = 0; ok && c0 < cols; c0 += stage_cols) {
         const size_t nc = std::min(stage_cols, cols - c0);
-        ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc, hipMemcpyHostToDevice, e->stream));
+        ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc,
+                                     db_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, setup));
         const size_t work = nc / 16 * e->KC * 64;
-        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, e->stream, stage,
+        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, setup, stage,
                            (size_t)rows, e->db, nc / 16, e->KC, c0 / 16);
-        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(e->stream));
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(setup));
     }
     // packing matrix: one byte plane at a time
     for (int j = 0; ok && j < 4; j++) {
-        ok = HIP_OK(hipMemcpyAsync(stage, ub + (size_t)j * e->kp * cols, e->kp * cols, hipMemcpyHostToDevice, e->stream));
+        ok = HIP_OK(hipMemcpyAsync(stage, ub + (size_t)j * e->kp * cols, e->kp * cols, hipMemcpyHostToDevice, setup));
         const size_t work = e->CT * e->KCp * 64;
-        hipLaunchKernelGGL(tile_u_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, e->stream,
+        hipLaunchKernelGGL(tile_u_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, setup,
                            (const int8_t *)stage, e->u, e->CT, e->KCp, j);
-        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(e->stream));
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(setup));
     }
     if (stage) (void)hipFree(stage);
     if (ok) {
-        hipLaunchKernelGGL(u_colsum_kernel, dim3((unsigned)e->CT), dim3(256), 0, e->stream, (const int8_t *)e->u,
+        hipLaunchKernelGGL(u_colsum_kernel, dim3((unsigned)e->CT), dim3(256), 0, setup, (const int8_t *)e->u,
                            e->ucorr, e->KCp, e->k128e);
         ok = HIP_OK(hipGetLastError());
     }
@@ -491,7 +592,7 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
         ok = HIP_OK(hipMemcpy(e->that, th.data(), th.size() * 4, hipMemcpyHostToDevice)) &&
              HIP_OK(hipMemcpy(e->wtab, w.data(), n * 4, hipMemcpyHostToDevice)) &&
              HIP_OK(hipMemcpy(e->prefix, pre.data(), pre.size() * 4, hipMemcpyHostToDevice)) &&
-             HIP_OK(hipStreamSynchronize(e->stream));
+             HIP_OK(hipStreamSynchronize(setup));
     }
     if (!ok) {
         const std::string msg = g_err;
@@ -500,6 +601,22 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
         return nullptr;
     }
     return e;
+}
+
+void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld, size_t rows, size_t cols,
+                      const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len,
+                      size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
+                      const uint64_t *inv_table, uint64_t q, int threads) {
+    return engine_create(db_col, false, db_tiled, ld, rows, cols, ub, n, tg, prefix, part_len, row1_off, q1, q1_bits,
+                         that, inv_table, q, threads);
+}
+
+void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t rows, size_t cols, const int8_t *ub,
+                             size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len, size_t row1_off,
+                             uint64_t q1, int q1_bits, const uint64_t *const *that, const uint64_t *inv_table,
+                             uint64_t q, int threads) {
+    return engine_create(db_col_device, true, db_col_device, ld, rows, cols, ub, n, tg, prefix, part_len, row1_off,
+                         q1, q1_bits, that, inv_table, q, threads);
 }
 
 void alto_engine_free(void *p) { delete (Engine *)p; }
@@ -511,20 +628,58 @@ size_t alto_engine_device_bytes(const void *p) { return ((const Engine *)p)->dev
 int alto_answer_device(void *p, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out, size_t out_stride,
                        int *status, void *stream) {
     auto *e = (Engine *)p;
-    if (!e || !slots || !out || !status || ((uintptr_t)slots & 15) || (slot_stride & 15) ||
-        slot_stride < ALTO_FRAME_OFFSET + e->request_len() || out_stride < e->response_len()) {
+    if (!e) {
+        set_err("alto_answer_device: null engine");
+        return -1;
+    }
+    return alto_answer_gather(e->ctx0, slots, slot_stride, nullptr, B, out, out_stride, nullptr, status, stream);
+}
+
+void *alto_context_new(void *p) {
+    auto *e = (Engine *)p;
+    if (!e) return nullptr;
+    if (!HIP_OK(hipSetDevice(e->device))) return nullptr;
+    auto *c = new Context;
+    if (!c->init(e)) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void alto_context_free(void *c) {
+    auto *ctx = (Context *)c;
+    if (ctx && ctx->e && ctx == ctx->e->ctx0) return;  // the default context lives with its engine
+    delete ctx;
+}
+
+void *alto_engine_context(void *p) { return p ? ((Engine *)p)->ctx0 : nullptr; }
+
+int alto_context_reserve(void *c, size_t B) {
+    auto *ctx = (Context *)c;
+    if (!ctx) return -1;
+    if (!HIP_OK(hipSetDevice(ctx->e->device))) return -1;
+    return ctx->reserve(B, false) ? 0 : -4;
+}
+
+int alto_answer_gather(void *c, const uint8_t *slots, size_t slot_stride, const int *slot_index, size_t B,
+                       uint8_t *out, size_t out_stride, const int *out_index, int *status, void *stream) {
+    auto *ctx = (Context *)c;
+    if (!ctx || !slots || !out || !status || ((uintptr_t)slots & 15) || (slot_stride & 15) ||
+        slot_stride < ALTO_FRAME_OFFSET + ctx->e->request_len() || out_stride < ctx->e->response_len()) {
         set_err("alto_answer_device: bad arguments");
         return -1;
     }
+    Engine *e = ctx->e;
     if (!B) return 0;
     if (!e->tail_ok) {
         set_err("unsupported tail configuration");
         return -3;
     }
     if (!HIP_OK(hipSetDevice(e->device))) return -1;
-    if (!e->reserve(B, false)) return -4;
-    hipStream_t st = stream ? (hipStream_t)stream : e->stream;
-    return e->enqueue(slots, slot_stride, B, out, out_stride, status, st, true);
+    if (!ctx->reserve(B, false)) return -4;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return ctx->answer_device(FrameSet{slots, slot_stride, slot_index}, B, out, out_stride, out_index, status, st);
 }
 
 int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B, uint8_t *const *out, int *status,
@@ -538,22 +693,24 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     if (stage_ms) std::fill(stage_ms, stage_ms + 4, 0.0);
     if (!B) return 0;
     if (!HIP_OK(hipSetDevice(e->device))) return -1;
-    if (!e->reserve(B, true)) return -1;
-    const size_t want = e->request_len(), rl = e->response_len(), S = e->ws.slot_stride;
-    e->hstatus.assign(B, 0);
-    for (size_t c = 0; c < B; c++) e->hstatus[c] = (!req[c] || len[c] != want) ? 1 : 0;
-    hipStream_t st = e->stream;
-    bool ok = HIP_OK(hipEventRecord(e->ev[0], st));
+    Context *ctx = e->ctx0;
+    if (!ctx->reserve(B, true)) return -1;
+    const size_t want = e->request_len(), rl = e->response_len(), S = ctx->slot_stride;
+    ctx->hstatus.assign(B, 0);
+    for (size_t c = 0; c < B; c++) ctx->hstatus[c] = (!req[c] || len[c] != want) ? 1 : 0;
+    hipStream_t st = ctx->stream;
+    const FrameSet fs{ctx->slots, S, nullptr};
+    bool ok = HIP_OK(hipEventRecord(ctx->ev[0], st));
     for (size_t c = 0; ok && c < B; c++)
-        if (!e->hstatus[c])
-            ok = HIP_OK(hipMemcpyAsync(e->ws.slots + c * S + ALTO_FRAME_OFFSET, req[c], want, hipMemcpyHostToDevice, st));
-    ok = ok && HIP_OK(hipMemcpyAsync(e->ws.status, e->hstatus.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
+        if (!ctx->hstatus[c])
+            ok = HIP_OK(hipMemcpyAsync(ctx->slots + c * S + ALTO_FRAME_OFFSET, req[c], want, hipMemcpyHostToDevice, st));
+    ok = ok && HIP_OK(hipMemcpyAsync(ctx->status, ctx->hstatus.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
     if (!ok) return -1;
     if (!e->tail_ok) {
         // validation only: the reference fails the whole batch with -3 once any request is valid
-        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, e->ws.slots, S, e->ws.status,
-                           (uint32_t)e->kp, e->ws.qsum, e->ws.ysum);
-        if (!HIP_OK(hipMemcpyAsync(status, e->ws.status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, ctx->status, (uint32_t)e->kp,
+                           ctx->qsum, ctx->ysum);
+        if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
             !HIP_OK(hipStreamSynchronize(st)))
             return -1;
         for (size_t c = 0; c < B; c++)
@@ -561,28 +718,36 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
         return 0;
     }
     // the upload is timed as part of the validation stage (ev[0] was recorded before it)
-    if (e->enqueue(e->ws.slots, S, B, e->ws.out, rl, e->ws.status, st, true, false)) return -1;
-    if (!HIP_OK(hipMemcpyAsync(status, e->ws.status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
+    if (ctx->enqueue(fs, B, ctx->out, rl, nullptr, ctx->status, st, true, false)) return -1;
+    if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
         !HIP_OK(hipStreamSynchronize(st)))
         return -1;
     int answered = 0;
     for (size_t c = 0; c < B; c++)
         if (status[c] == 0) {
             answered++;
-            if (!HIP_OK(hipMemcpyAsync(out[c], e->ws.out + c * rl, rl, hipMemcpyDeviceToHost, st))) return -1;
+            if (!HIP_OK(hipMemcpyAsync(out[c], ctx->out + c * rl, rl, hipMemcpyDeviceToHost, st))) return -1;
         }
     // the download is timed as part of the tail stage
-    if (!HIP_OK(hipEventRecord(e->ev[6], st)) || !HIP_OK(hipStreamSynchronize(st))) return -1;
+    if (!HIP_OK(hipEventRecord(ctx->ev[6], st)) || !HIP_OK(hipStreamSynchronize(st))) return -1;
     if (stage_ms) {
         // {validate, scan, pack, tail} as in src/amx_kernels.cpp:613
         const int from[4] = {0, 1, 3, 5}, to[4] = {1, 3, 5, 6};
         for (int i = 0; i < 4; i++) {
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, e->ev[from[i]], e->ev[to[i]]);
+            (void)hipEventElapsedTime(&ms, ctx->ev[from[i]], ctx->ev[to[i]]);
             stage_ms[i] = ms;
         }
     }
     return answered;
+}
+
+int alto_set_graphs(void *p, int on) {
+    auto *e = (Engine *)p;
+    if (!e) return -1;
+    if (!on) e->ctx0->drop_graphs();
+    e->use_graphs = on != 0;
+    return 0;
 }
 
 int alto_variant_count(void) { return kNumVariants; }
@@ -602,17 +767,19 @@ int alto_tune_get(const void *p, int which, size_t B) {
     return e->variant_for(which, (int)(round_up(B, 4) / 4));
 }
 
-int alto_last_timings(void *p, double *ms, int count) {
-    auto *e = (Engine *)p;
-    if (!e || !ms || count <= 0 || !e->timed_valid) return 0;
-    if (!HIP_OK(hipEventSynchronize(e->ev[6]))) return -1;
+int alto_context_timings(void *c, double *ms, int count) {
+    auto *ctx = (Context *)c;
+    if (!ctx || !ms || count <= 0 || !ctx->timed_valid) return 0;
+    if (!HIP_OK(hipEventSynchronize(ctx->ev[6]))) return -1;
     const int k = std::min(count, 6);
     for (int i = 0; i < k; i++) {
         float f = 0;
-        (void)hipEventElapsedTime(&f, e->ev[i], e->ev[i + 1]);
+        (void)hipEventElapsedTime(&f, ctx->ev[i], ctx->ev[i + 1]);
         ms[i] = f;
     }
     return k;
 }
+
+int alto_last_timings(void *p, double *ms, int count) { return p ? alto_context_timings(((Engine *)p)->ctx0, ms, count) : 0; }
 
 }  // extern "C"

@@ -136,6 +136,19 @@ __global__ __launch_bounds__(256) void u_colsum_kernel(const int8_t *__restrict_
 
 // ------------------------------------------------------------ per batch: checks ----
 
+// Where the frames of a batch live: frame c (batch position) is slot sidx[c] of the arena,
+// or slot c when no index array is given; each slot holds one frame at byte 7 so that the
+// u32 payload starting at frame byte 9 is 16-byte aligned.
+struct FrameSet {
+    const uint8_t *slots;
+    size_t stride;
+    const int *sidx;
+    __device__ __forceinline__ const uint8_t *slot(size_t c) const {
+        return slots + (size_t)(sidx ? sidx[c] : (int)c) * stride;
+    }
+};
+
+
 __device__ __forceinline__ uint32_t ld32_bytes(const uint8_t *p) {
     return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
 }
@@ -143,8 +156,7 @@ __device__ __forceinline__ uint32_t ld32_bytes(const uint8_t *p) {
 // Request validation (src/amx_kernels.cpp:571-581): one workgroup per frame.  Frames with
 // a nonzero status on entry (rejected by the host: null or wrong length) keep it.
 // Also clears the per-query sums that the prep kernels accumulate.
-__global__ __launch_bounds__(256) void validate_kernel(const uint8_t *__restrict__ slots, size_t stride,
-                                                       int *__restrict__ status, uint32_t kp,
+__global__ __launch_bounds__(256) void validate_kernel(FrameSet fs, int *__restrict__ status, uint32_t kp,
                                                        uint32_t *__restrict__ qsum,
                                                        unsigned long long *__restrict__ ysum) {
     __shared__ int st;
@@ -156,7 +168,7 @@ __global__ __launch_bounds__(256) void validate_kernel(const uint8_t *__restrict
     }
     __syncthreads();
     if (st != 0) return;
-    const uint8_t *f = slots + c * stride + 7;
+    const uint8_t *f = fs.slot(c) + 7;
     if (f[0] != 0 || ld32_bytes(f + 1) != kp || ld32_bytes(f + 5) != kp) {
         if (threadIdx.x == 0) status[c] = 2;
         return;
@@ -195,8 +207,7 @@ __device__ __forceinline__ T block_sum(T v, T *red) {
 // PACK: bytes of y offset by -128 and the exact sum(y) into ysum.
 // grid = (ceil(KC * 4 / 256), Bp); clients c >= B or with status != 0 get zero rows.
 template <bool SCAN>
-__global__ __launch_bounds__(256) void limb_prep_kernel(const uint8_t *__restrict__ slots, size_t stride,
-                                                        size_t word_off, const int *__restrict__ status, int B,
+__global__ __launch_bounds__(256) void limb_prep_kernel(FrameSet fs, size_t word_off, const int *__restrict__ status, int B,
                                                         size_t KC, int8_t *__restrict__ dst,
                                                         uint32_t *__restrict__ qsum,
                                                         unsigned long long *__restrict__ ysum) {
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(256) void limb_prep_kernel(const uint8_t *__restric
     if (kc < KC) {
         uint4 o[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
         if (ok) {
-            const uint4 *src = (const uint4 *)(slots + (size_t)c * stride + 16 + word_off * 4) + (kc * 64 + g * 16) / 4;
+            const uint4 *src = (const uint4 *)(fs.slot(c) + 16 + word_off * 4) + (kc * 64 + g * 16) / 4;
             uint32_t w[16];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -589,8 +600,7 @@ __global__ __launch_bounds__(256, 1) void dma_kernel(const i32x4 *__restrict__ a
 // ---------------------------------------------------------------------------- tail --
 
 struct TailArgs {
-    const uint8_t *slots;
-    size_t stride;
+    FrameSet fs;
     const int *status;
     const uint32_t *acc;    // scan accumulators, B x C
     const uint32_t *qsum;   // per query sum(q) mod 2^32
@@ -600,6 +610,7 @@ struct TailArgs {
     const uint8_t *prefix;  // O x part_words * 4 (each prefix zero-padded to whole words)
     uint8_t *out;
     size_t out_stride;
+    const int *oidx;        // output slot of each batch position (null: position itself)
     int n, logn, tg, C, kp;
     int part_len, part_words;
     long long row1_off;
@@ -651,10 +662,10 @@ __device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *(const uint4 *
 // NTT (src/amx_kernels.cpp:453-513), row-1 switch and LSB-first bit packing ORed onto the
 // precomputed prefix (src/amx_kernels.cpp:515-537, 604-607).
 //
-// The transform runs in radix-8 phases: in phase p a thread holds a group of 8 elements
-// base + 8^p * m (m < 8) in registers and applies the (up to) three stages t = 8^p, 2*8^p,
-// 4*8^p that stay inside the group; groups are exchanged through LDS between phases, so
-// n = 2048 needs three barriers instead of eleven.  Phase 0 takes its group straight from
+// The transform runs in radix-8 phases: in a phase a thread holds a group of 8 elements
+// base + 2^ls * m (m < 8) in registers and applies the stages t = 2^ls, 2^(ls+1), 2^(ls+2)
+// that stay inside the group (phase 0 only the first 1-3 stages); groups are exchanged
+// through LDS between phases, so n = 2048 needs three barriers instead of eleven.  Phase 0 takes its group straight from
 // the pointwise product and the last phase feeds the switch directly.
 __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     extern __shared__ uint32_t lds[];
@@ -666,14 +677,17 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     const size_t col0 = (size_t)o * n;
     const uint32_t *pre = (const uint32_t *)(p.prefix + (size_t)o * p.part_words * 4);
     for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) part[w] = pre[w];
-    const uint32_t *zk = (const uint32_t *)(p.slots + (size_t)c * p.stride + 16) + p.kp;
+    const uint32_t *zk = (const uint32_t *)(p.fs.slot(c) + 16) + p.kp;
     const uint32_t *th = p.that + (size_t)o * p.kp;
     const uint32_t *bc = p.beta + (size_t)c * p.C + col0;
     const uint32_t *accc = p.acc + (size_t)c * p.C + col0;
     const uint32_t qs = p.qsum[c] << 7;
-    const int nph = (logn + 2) / 3;
+    // phase 0 takes the r0 = logn - 3 (nph - 1) first stages (1..3) on contiguous groups, so
+    // that every later phase has exactly three stages and its groups (stride 2^ls, 8
+    // elements) stay inside the n points
+    const int nph = (logn + 2) / 3, r0 = logn - 3 * (nph - 1);
     for (int ph = 0; ph < nph; ph++) {
-        const int ls = 3 * ph, sp = 1 << ls;
+        const int ls = ph == 0 ? 0 : r0 + 3 * (ph - 1), sp = 1 << ls, nst = ph == 0 ? r0 : 3;
         for (int g = threadIdx.x; g < groups; g += blockDim.x) {
             const int base = ((g >> ls) << (ls + 3)) | (g & (sp - 1));
             uint32_t v[8];
@@ -699,8 +713,8 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
             }
 #pragma unroll
             for (int q = 0; q < 3; q++) {
+                if (q >= nst) break;
                 const int lt = ls + q;
-                if (lt >= logn) break;
                 const int h = n >> (lt + 1);
 #pragma unroll
                 for (int m = 0; m < 8; m++) {
@@ -731,7 +745,7 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
         }
         __syncthreads();
     }
-    uint8_t *dst = p.out + (size_t)c * p.out_stride + (size_t)o * p.part_len;
+    uint8_t *dst = p.out + (size_t)(p.oidx ? p.oidx[c] : c) * p.out_stride + (size_t)o * p.part_len;
     if ((((uintptr_t)dst) & 3) == 0 && (p.part_len & 3) == 0) {
         uint32_t *d32 = (uint32_t *)dst;
         for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) d32[w] = part[w];

@@ -1,0 +1,366 @@
+// alto_serve.hip -- device-resident request loop with continuous batching.
+//
+// The reference serves requests with reader/writer threads around one engine thread that
+// drains the whole queue into one synchronous alto_answer call per batch
+// (src/bin/altopir.rs:372-496).  Here every resident database keeps a pool of request
+// slots (pinned host staging + a device arena) and several engine contexts:
+//   submit   copies the frame into a free pinned slot and starts its upload at once on the
+//            database's upload stream (an event per slot marks its arrival in HBM);
+//   dispatch whenever a context is free, takes every waiting request of that database (up
+//            to max_batch, never waiting to fill a batch), makes the context's stream wait
+//            for their uploads and answers them with alto_answer_gather; responses are
+//            written by the tail kernel straight into mapped pinned host memory;
+//   complete a completion thread waits for each batch, stamps the completion time and
+//            hands the tickets to poll().
+// Batches of one database overlap on the device as far as its contexts allow; the number
+// of contexts is bounded by the device memory free at start (memory admission).  Frames
+// whose length is wrong get status 1 at once, and status 9 when every slot is taken
+// (src/bin/altopir.rs:417-436); the device validates the rest (statuses 2 and 3).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/altopir_engine.h"
+
+namespace {
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+struct Server;
+
+struct Ctx {
+    void *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int *d_idx = nullptr;   // device: slot index of each batch position
+    int *h_idx = nullptr;   // pinned staging of d_idx
+    int *h_status = nullptr;  // mapped pinned: statuses written by the device
+    int *d_status = nullptr;  // device view of h_status
+    std::vector<int> batch;   // slots of the batch in flight
+    bool busy = false;
+};
+
+struct Db {
+    void *engine = nullptr;
+    size_t L = 0, RL = 0, S = 0;
+    int nslots = 0;
+    uint8_t *h_req = nullptr, *d_req = nullptr;      // nslots x S
+    uint8_t *h_resp = nullptr, *d_resp = nullptr;    // nslots x RL, mapped pinned
+    std::vector<hipEvent_t> slot_ev;
+    std::vector<int> free_slots;
+    std::deque<int> pending;
+    std::vector<uint64_t> tag;
+    std::vector<int64_t> t_submit;
+    hipStream_t upload = nullptr;
+    std::vector<Ctx> ctx;
+    std::mutex up_mu;  // serialises uploads on this database's stream
+};
+
+struct Server {
+    std::vector<std::unique_ptr<Db>> dbs;
+    int max_batch = 128;
+    std::mutex mu;
+    std::condition_variable cv_dispatch, cv_done, cv_complete;
+    std::deque<alto_completion> completed;
+    std::deque<std::pair<int, int>> inflight;  // (db, ctx) in launch order
+    std::atomic<bool> stop{false};
+    std::vector<std::thread> threads;
+    std::string err;
+    alto_server_stats stats{};
+    int device = 0;
+};
+
+bool ok(hipError_t e, Server *s, const char *what) {
+    if (e == hipSuccess) return true;
+    s->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+void dispatcher(Server *s, int di) {
+    (void)hipSetDevice(s->device);
+    Db &d = *s->dbs[di];
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        int ci = -1;
+        s->cv_dispatch.wait(lk, [&] {
+            if (s->stop) return true;
+            if (d.pending.empty()) return false;
+            for (size_t i = 0; i < d.ctx.size(); i++)
+                if (!d.ctx[i].busy) {
+                    ci = (int)i;
+                    return true;
+                }
+            return false;
+        });
+        if (s->stop) return;
+        Ctx &c = d.ctx[ci];
+        c.batch.clear();
+        while (!d.pending.empty() && (int)c.batch.size() < s->max_batch) {
+            c.batch.push_back(d.pending.front());
+            d.pending.pop_front();
+        }
+        c.busy = true;
+        const int B = (int)c.batch.size();
+        lk.unlock();
+        for (int i = 0; i < B; i++) {
+            c.h_idx[i] = c.batch[i];
+            c.h_status[i] = 0;
+            (void)hipStreamWaitEvent(c.stream, d.slot_ev[c.batch[i]], 0);
+        }
+        bool good = ok(hipMemcpyAsync(c.d_idx, c.h_idx, B * sizeof(int), hipMemcpyHostToDevice, c.stream), s, "index");
+        const int rc = good ? alto_answer_gather(c.ctx, d.d_req, d.S, c.d_idx, B, d.d_resp, d.RL, c.d_idx, c.d_status,
+                                                 c.stream)
+                            : -1;
+        if (rc != 0) s->err = std::string("alto_answer_gather: ") + alto_last_error();
+        (void)hipEventRecord(c.done, c.stream);
+        lk.lock();
+        if (rc != 0)
+            for (int i = 0; i < B; i++) c.h_status[i] = -1;  // reported as an engine failure
+        s->inflight.emplace_back(di, ci);
+        s->stats.batches++;
+        s->stats.answered_in_batches += B;
+        s->stats.max_batch_seen = std::max<long long>(s->stats.max_batch_seen, B);
+        s->cv_done.notify_one();
+    }
+}
+
+void completer(Server *s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        s->cv_done.wait(lk, [&] { return s->stop || !s->inflight.empty(); });
+        if (s->inflight.empty() && s->stop) return;
+        auto [di, ci] = s->inflight.front();
+        s->inflight.pop_front();
+        Db &d = *s->dbs[di];
+        Ctx &c = d.ctx[ci];
+        lk.unlock();
+        (void)hipEventSynchronize(c.done);
+        const int64_t t = now_ns();
+        lk.lock();
+        const int B = (int)c.batch.size();
+        for (int i = 0; i < B; i++) {
+            const int slot = c.batch[i];
+            alto_completion x{};
+            x.tag = d.tag[slot];
+            x.db = di;
+            x.status = c.h_status[i];
+            x.slot = x.status == 0 ? slot : -1;
+            x.response = x.status == 0 ? d.h_resp + (size_t)slot * d.RL : nullptr;
+            x.response_len = x.status == 0 ? d.RL : 0;
+            x.t_submit_ns = d.t_submit[slot];
+            x.t_done_ns = t;
+            x.batch = B;
+            if (x.status != 0) d.free_slots.push_back(slot);  // nothing to read back
+            s->completed.push_back(x);
+        }
+        c.busy = false;
+        s->cv_complete.notify_all();
+        s->cv_dispatch.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void *alto_server_new(void *const *engines, int n_engines, int max_batch, int slots_per_db, int contexts_per_db) {
+    if (!engines || n_engines <= 0 || max_batch <= 0 || slots_per_db <= 0 || contexts_per_db <= 0) return nullptr;
+    auto *s = new Server;
+    s->max_batch = max_batch;
+    (void)hipGetDevice(&s->device);
+    for (int i = 0; i < n_engines; i++) s->dbs.emplace_back(new Db);
+    bool good = true;
+    for (int i = 0; good && i < n_engines; i++) {
+        Db &d = *s->dbs[i];
+        d.engine = engines[i];
+        d.L = alto_request_len(d.engine);
+        d.RL = alto_response_len(d.engine);
+        d.S = round_up(ALTO_FRAME_OFFSET + d.L, 256);
+        d.nslots = slots_per_db;
+        good = ok(hipHostMalloc((void **)&d.h_req, (size_t)d.nslots * d.S, hipHostMallocDefault), s, "pinned requests") &&
+               ok(hipMalloc(&d.d_req, (size_t)d.nslots * d.S), s, "request arena") &&
+               ok(hipHostMalloc((void **)&d.h_resp, (size_t)d.nslots * d.RL, hipHostMallocMapped), s, "responses") &&
+               ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping") &&
+               ok(hipStreamCreateWithFlags(&d.upload, hipStreamNonBlocking), s, "upload stream");
+        d.slot_ev.resize(d.nslots);
+        for (auto &ev : d.slot_ev) good = good && ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), s, "event");
+        d.tag.resize(d.nslots);
+        d.t_submit.resize(d.nslots);
+        for (int k = d.nslots - 1; k >= 0; k--) d.free_slots.push_back(k);
+        // memory admission: as many contexts (each reserved for max_batch queries) as fit in
+        // 90% of the device memory free now, at least one, at most contexts_per_db
+        for (int k = 0; good && k < contexts_per_db; k++) {
+            size_t free_b = 0, total_b = 0;
+            (void)hipMemGetInfo(&free_b, &total_b);
+            Ctx c;
+            c.ctx = alto_context_new(d.engine);
+            if (!c.ctx) {
+                good = k > 0;
+                break;
+            }
+            if (alto_context_reserve(c.ctx, max_batch) != 0) {
+                alto_context_free(c.ctx);
+                good = k > 0;
+                if (!good) s->err = "not enough device memory for one batch of max_batch queries";
+                break;
+            }
+            size_t free_after = 0;
+            (void)hipMemGetInfo(&free_after, &total_b);
+            good = ok(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), s, "stream") &&
+                   ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), s, "event") &&
+                   ok(hipMalloc(&c.d_idx, max_batch * sizeof(int)), s, "index") &&
+                   ok(hipHostMalloc((void **)&c.h_idx, max_batch * sizeof(int), hipHostMallocDefault), s, "index") &&
+                   ok(hipHostMalloc((void **)&c.h_status, max_batch * sizeof(int), hipHostMallocMapped), s, "status") &&
+                   ok(hipHostGetDevicePointer((void **)&c.d_status, c.h_status, 0), s, "status mapping");
+            d.ctx.push_back(c);
+            const size_t used = free_b > free_after ? free_b - free_after : 0;
+            if (free_after < used + free_b / 10) break;
+        }
+    }
+    if (!good) {
+        std::string e = s->err;
+        alto_server_free(s);
+        fprintf(stderr, "alto_server_new: %s\n", e.c_str());
+        return nullptr;
+    }
+    for (int i = 0; i < n_engines; i++) s->threads.emplace_back(dispatcher, s, i);
+    s->threads.emplace_back(completer, s);
+    return s;
+}
+
+int alto_server_contexts(void *p, int db) {
+    auto *s = (Server *)p;
+    if (!s || db < 0 || db >= (int)s->dbs.size()) return -1;
+    return (int)s->dbs[db]->ctx.size();
+}
+
+int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64_t tag) {
+    auto *s = (Server *)p;
+    if (!s || db < 0 || db >= (int)s->dbs.size()) return -1;
+    Db &d = *s->dbs[db];
+    const int64_t t0 = now_ns();
+    int slot = -1;
+    int reject = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.submitted++;
+        if (!frame || len != d.L) reject = 1;
+        else if (d.free_slots.empty()) reject = 9;
+        else {
+            slot = d.free_slots.back();
+            d.free_slots.pop_back();
+        }
+        if (reject) {
+            alto_completion x{};
+            x.tag = tag;
+            x.db = db;
+            x.status = reject;
+            x.slot = -1;
+            x.t_submit_ns = t0;
+            x.t_done_ns = now_ns();
+            s->completed.push_back(x);
+            if (reject == 9) s->stats.queue_full++;
+            s->cv_complete.notify_all();
+            return reject;
+        }
+    }
+    uint8_t *h = d.h_req + (size_t)slot * d.S;
+    memcpy(h + ALTO_FRAME_OFFSET, frame, len);
+    {
+        std::lock_guard<std::mutex> ul(d.up_mu);
+        (void)hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, d.S, hipMemcpyHostToDevice, d.upload);
+        (void)hipEventRecord(d.slot_ev[slot], d.upload);
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        d.tag[slot] = tag;
+        d.t_submit[slot] = t0;
+        d.pending.push_back(slot);
+    }
+    s->cv_dispatch.notify_all();
+    return 0;
+}
+
+int alto_server_poll(void *p, alto_completion *out, int max, int timeout_ms) {
+    auto *s = (Server *)p;
+    if (!s || !out || max <= 0) return -1;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->completed.empty() && timeout_ms > 0)
+        s->cv_complete.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->completed.empty(); });
+    int k = 0;
+    while (k < max && !s->completed.empty()) {
+        out[k++] = s->completed.front();
+        s->completed.pop_front();
+    }
+    return k;
+}
+
+int alto_server_release(void *p, int db, int slot) {
+    auto *s = (Server *)p;
+    if (!s || db < 0 || db >= (int)s->dbs.size() || slot < 0 || slot >= s->dbs[db]->nslots) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->dbs[db]->free_slots.push_back(slot);
+    s->cv_dispatch.notify_all();
+    return 0;
+}
+
+int alto_server_get_stats(void *p, alto_server_stats *out) {
+    auto *s = (Server *)p;
+    if (!s || !out) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return 0;
+}
+
+const char *alto_server_error(void *p) { return p ? ((Server *)p)->err.c_str() : ""; }
+
+void alto_server_free(void *p) {
+    auto *s = (Server *)p;
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_dispatch.notify_all();
+    s->cv_done.notify_all();
+    s->cv_complete.notify_all();
+    for (auto &t : s->threads) t.join();
+    for (auto &dp : s->dbs) {
+        Db &d = *dp;
+        for (auto &c : d.ctx) {
+            if (c.stream) (void)hipStreamSynchronize(c.stream);
+            if (c.ctx) alto_context_free(c.ctx);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+            if (c.done) (void)hipEventDestroy(c.done);
+            if (c.d_idx) (void)hipFree(c.d_idx);
+            if (c.h_idx) (void)hipHostFree(c.h_idx);
+            if (c.h_status) (void)hipHostFree(c.h_status);
+        }
+        if (d.upload) (void)hipStreamSynchronize(d.upload);
+        for (auto &ev : d.slot_ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (d.upload) (void)hipStreamDestroy(d.upload);
+        if (d.h_req) (void)hipHostFree(d.h_req);
+        if (d.d_req) (void)hipFree(d.d_req);
+        if (d.h_resp) (void)hipHostFree(d.h_resp);
+    }
+    delete s;
+}
+
+}  // extern "C"

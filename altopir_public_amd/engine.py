@@ -170,6 +170,24 @@ class Engine:
         return cls(inp.db_col, tiled, inp.ld, inp.ub, inp.n, inp.tg, inp.prefixes, inp.row1_off, inp.q1,
                    inp.q1_bits, inp.that, inp.inv_table, Q, threads)
 
+    @classmethod
+    def from_device_db(cls, db_ptr: int, inp, threads: int = 8) -> "Engine":
+        """Engine over a column-major database already in device memory (db_ptr, ld =
+        inp.ld); the other setup inputs come from workload.EngineInputs."""
+        self = cls.__new__(cls)
+        self._lib = _lib.load()
+        prefixes = [np.ascontiguousarray(p, dtype=np.uint8) for p in inp.prefixes]
+        that = [np.ascontiguousarray(t, dtype=np.uint64) for t in inp.that]
+        inv_table = np.ascontiguousarray(inp.inv_table, dtype=np.uint64)
+        pp = (C.c_void_p * len(prefixes))(*[_ptr(p) for p in prefixes])
+        tp = (C.c_void_p * len(that))(*[_ptr(t) for t in that])
+        self.ptr = self._lib.alto_engine_new_device(db_ptr, inp.ld, inp.rows, inp.cols, _ptr(inp.ub), inp.n, inp.tg,
+                                                    C.addressof(pp), inp.part_len, inp.row1_off, inp.q1, inp.q1_bits,
+                                                    C.addressof(tp), _ptr(inv_table), Q, threads)
+        assert self.ptr, f"unsupported engine configuration ({_lib.last_error()})"
+        self.rows, self.cols = inp.rows, inp.cols
+        return self
+
     def request_len(self) -> int:
         return self._lib.alto_request_len(self.ptr)
 
@@ -217,6 +235,10 @@ class Engine:
         """Device-resident batch (alto_answer_device): all pointers are device addresses."""
         return self._lib.alto_answer_device(self.ptr, slots_ptr, slot_stride, b, out_ptr, out_stride, status_ptr,
                                             stream)
+
+    def set_graphs(self, on: bool) -> None:
+        """Graph capture/replay of device-resident batches (alto_set_graphs)."""
+        self._lib.alto_set_graphs(self.ptr, 1 if on else 0)
 
     def last_timings(self) -> list:
         ms = (C.c_double * 6)()

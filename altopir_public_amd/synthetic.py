@@ -54,6 +54,31 @@ def uniform_bytes(seed: int, nbytes: int) -> np.ndarray:
     return stream(seed, (nbytes + 7) // 8).view(np.uint8)[:nbytes]
 
 
+def uniform_bytes_device(seed: int, nbytes: int, device="cuda", chunk: int = 1 << 27):
+    """uniform_bytes(seed, nbytes) generated on the GPU with torch (int64 arithmetic wraps
+    like u64; the logical right shifts are arithmetic shifts masked to their width)."""
+    import torch
+
+    words = (nbytes + 7) // 8
+    out = torch.empty(words * 8, dtype=torch.uint8, device=device)
+    w64 = out.view(torch.int64)
+    s64 = lambda v: v - (1 << 64) if v >= (1 << 63) else v  # noqa: E731
+    seed64 = s64(seed & ((1 << 64) - 1))
+    g, m1, m2 = s64(0x9E3779B97F4A7C15), s64(0xBF58476D1CE4E5B9), s64(0x94D049BB133111EB)
+    for lo in range(0, words, chunk):
+        hi = min(words, lo + chunk)
+        z = torch.arange(lo + 1, hi + 1, dtype=torch.int64, device=device)
+        z.mul_(g).add_(seed64)
+        z.bitwise_xor_((z >> 30) & ((1 << 34) - 1))
+        z.mul_(m1)
+        z.bitwise_xor_((z >> 27) & ((1 << 37) - 1))
+        z.mul_(m2)
+        z.bitwise_xor_((z >> 31) & ((1 << 33) - 1))
+        w64[lo:hi] = z
+        del z
+    return out[:nbytes]
+
+
 def uniform_u32(seed: int, count: int) -> np.ndarray:
     """count i.i.d. uniform u32 values (low halves of the stream)."""
     return (stream(seed, count) & np.uint64(0xFFFFFFFF)).astype(np.uint32)

@@ -100,7 +100,7 @@ class EngineInputs:
 
 
 def make_inputs(rows: int, cols: int, n: int = 2048, tg: int = 4, seed: int = 0, ld: int | None = None,
-                table: str = "ntt", prefix: str = "row0", db: np.ndarray | None = None) -> EngineInputs:
+                table: str = "ntt", prefix: str = "row0", db=None) -> EngineInputs:
     """Seeded setup artefacts.  db_col byte (col * ld + k) is byte (col * rows + k) of the
     stream `seed` (i.e. byte `col` of record `k`); U's tile bytes come from stream seed+1,
     `that` from seed+2 (full 64-bit values), random inverse tables from seed+3, prefixes
@@ -110,7 +110,9 @@ def make_inputs(rows: int, cols: int, n: int = 2048, tg: int = 4, seed: int = 0,
     every byte, which exercises the OR of row-1 fields onto the prefix."""
     ld = rows if ld is None else ld
     kp = tg * n
-    if db is None:
+    if db is False:
+        db_col = None  # the caller generates the database elsewhere (e.g. on the device)
+    elif db is None:
         body = synthetic.uniform_bytes(seed, rows * cols).reshape(cols, rows)
         if ld == rows:
             db_col = body.reshape(-1)
@@ -169,3 +171,34 @@ def fill_requests(arena: np.ndarray, B: int, rows: int, kp: int, seed: int) -> N
     L = request_len(rows, kp)
     for c, f in enumerate(make_requests(B, rows, kp, seed)):
         arena[c, :L] = f
+
+
+def frame_status(frame, rows: int, kp: int) -> int:
+    """The status the engine gives a frame (src/amx_kernels.cpp:571-581): 1 length, 2 flag or
+    key-length field, 3 a key word >= Q, 0 valid."""
+    if frame is None:
+        return 1
+    f = bytes(frame)
+    if len(f) != request_len(rows, kp):
+        return 1
+    if f[0] != 0 or int.from_bytes(f[1:5], "little") != kp or int.from_bytes(f[5:9], "little") != kp:
+        return 2
+    keys = np.frombuffer(f, dtype="<u4", count=2 * kp, offset=9)
+    return 3 if (keys >= Q).any() else 0
+
+
+def corrupt(frame, rng: np.random.Generator, kp: int) -> bytes:
+    """One corruption of a valid frame, as a client bug or an attacker would make it: a
+    truncated or extended frame, a flipped header byte, or a key word set to Q or above."""
+    f = bytearray(bytes(frame))
+    kind = int(rng.integers(4))
+    if kind == 0:
+        del f[int(rng.integers(1, len(f))):]
+    elif kind == 1:
+        f += bytes(int(rng.integers(1, 64)))
+    elif kind == 2:
+        f[int(rng.integers(9))] ^= 1 << int(rng.integers(8))
+    else:
+        w = 9 + 4 * int(rng.integers(2 * kp))
+        f[w:w + 4] = int(Q + int(rng.integers(12287))).to_bytes(4, "little")
+    return bytes(f)

@@ -86,6 +86,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -95,7 +97,7 @@ def main() -> int:
 
     import torch
 
-    from altopir_public_amd import workload
+    from altopir_public_amd import synthetic, workload
     from altopir_public_amd.engine import Engine
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
@@ -103,10 +105,16 @@ def main() -> int:
     R, C, B = args.records, args.record_bytes, args.batch
     n, tg = 2048, 4
     t0 = time.time()
-    inp = workload.make_inputs(R, C, n=n, tg=tg, seed=0)
+    # the database (uniform bytes of stream 0, column-major) is generated straight into
+    # device memory; the other setup inputs on the host
+    inp = workload.make_inputs(R, C, n=n, tg=tg, seed=0, db=False)
+    d_db = synthetic.uniform_bytes_device(0, R * C)
+    torch.cuda.synchronize()
     log(f"[bench] inputs {R} x {C} B generated in {time.time() - t0:.1f} s")
     t0 = time.time()
-    eng = Engine.from_inputs(inp, threads=args.cpu_threads)
+    eng = Engine.from_device_db(d_db.data_ptr(), inp, threads=args.cpu_threads)
+    del d_db
+    torch.cuda.empty_cache()
     log(f"[bench] engine set up in {time.time() - t0:.1f} s, {eng.device_bytes() / 2**30:.2f} GiB resident")
     L, rl = inp.request_len, inp.response_len
     stride = round_up(7 + L, 256)
@@ -147,11 +155,14 @@ def main() -> int:
     qps = B / (ms_per_step / 1e3)
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {qps:.1f} QPS")
 
-    # per-kernel device times (HIP events recorded by the engine around each launch)
+    # per-kernel device times (HIP events recorded by the engine around each launch; the
+    # engine records them with graph replay off)
+    eng.set_graphs(False)
     stages = []
     for _ in range(min(args.steps, 10)):
         step()
         stages.append(eng.last_timings())
+    eng.set_graphs(True)
     st = np.array(stages).mean(axis=0)
     names = ["validate", "scan_prep", "scan", "pack_prep", "pack", "tail"]
     stage_ms = {k: round(float(v), 4) for k, v in zip(names, st)}
@@ -176,9 +187,21 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes": byts}
 
+    if args.copy_calib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the
+        # counters of the engine's kernels to bytes
+        x = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 copies of {args.copy_calib} MiB")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
+        if inp.db_col is None:
+            inp.db_col = synthetic.uniform_bytes(0, R * C)  # the same bytes, on the host
         cpu = cpu_baseline(inp, reqs, args.cpu_threads, args.cpu_budget)
         log(f"[bench] cpu baseline {cpu}")
 

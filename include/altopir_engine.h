@@ -95,11 +95,47 @@ int alto_answer(void *e, const uint8_t *const *req, const size_t *len, size_t B,
 int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
                        size_t out_stride, int *status, void *stream);
 
-/* Device time in milliseconds of the stages of the last batch (synchronous or device
- * entry point), waiting for it to finish: {validate, scan limb preparation, scan
+/* Contexts: the per-batch state of an engine (workspace, stream, events, captured graphs).
+ * alto_answer and alto_answer_device use the engine's default context
+ * (alto_engine_context); more contexts let several batches of one engine be in flight at
+ * once (one batch per context at a time).  alto_context_reserve sizes a context's
+ * workspace for B queries ahead of time (0, or -4 when device memory is short). */
+void *alto_context_new(void *e);
+void alto_context_free(void *ctx);
+void *alto_engine_context(void *e);
+int alto_context_reserve(void *ctx, size_t B);
+
+/* alto_answer_device on a context, with frames and outputs gathered through optional
+ * device index arrays: frame c at slots + slot_index[c] * slot_stride + ALTO_FRAME_OFFSET,
+ * output c at out + out_index[c] * out_stride (null index = position c). */
+int alto_answer_gather(void *ctx, const uint8_t *slots, size_t slot_stride, const int *slot_index,
+                       size_t B, uint8_t *out, size_t out_stride, const int *out_index, int *status,
+                       void *stream);
+
+/* Stage timings of a context's last eager batch (same layout as alto_last_timings). */
+int alto_context_timings(void *ctx, double *ms, int count);
+
+/* Graph mode of alto_answer_device (on by default): the first batch with a given set of
+ * pointers, size, stream and kernel variants is captured into a HIP graph and every later
+ * identical batch replays it (at most 64 graphs per engine, least recently used dropped;
+ * growing the batch workspace drops them all).  Stage timings are recorded only with
+ * graphs off.  Returns 0, -1 on a null engine. */
+int alto_set_graphs(void *e, int on);
+
+/* Device time in milliseconds of the stages of the last batch (synchronous entry point, or
+ * device entry point with graphs off), waiting for it to finish: {validate, scan limb preparation, scan
  * contraction, packing limb preparation, packing contraction, tail}.  Writes up to
  * `count` values and returns how many were written (0 if no batch ran yet). */
 int alto_last_timings(void *e, double *ms, int count);
+
+/* alto_engine_new with the column-major database already in device memory
+ * (db_col_device[col * ld + k]); the engine tiles it into its own copy, after which the
+ * caller may free it.  The other arguments and checks are those of alto_engine_new. */
+void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t rows, size_t cols,
+                             const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,
+                             size_t part_len, size_t row1_off, uint64_t q1, int q1_bits,
+                             const uint64_t *const *that, const uint64_t *inv_table, uint64_t q,
+                             int threads);
 
 /* Kernel variants of the two contractions (scan and packing) and the per-engine tuning
  * table that picks one per batch-size bucket (buckets: ceil(log2(ceil(B / 4))))).
@@ -109,6 +145,46 @@ int alto_variant_count(void);
 const char *alto_variant_name(int idx);
 int alto_tune_set(void *e, int which, size_t B, int idx);
 int alto_tune_get(const void *e, int which, size_t B);
+
+/* ------------------------------------------------------------- serving loop -- */
+
+/* A device-resident request loop over one or more engines (resident databases) with
+ * continuous batching: submitted frames are uploaded at once into pinned/HBM slots, and
+ * whenever one of a database's contexts is free, every waiting request of that database
+ * (up to max_batch) is answered as one batch; batches of one database overlap on the
+ * device as far as its contexts allow.  The number of contexts per database is at most
+ * contexts_per_db and at most what fits in the device memory free at creation (each is
+ * reserved for max_batch queries).  Responses are written by the device straight into
+ * pinned host memory and stay valid until alto_server_release. */
+typedef struct {
+    uint64_t tag;          /* the caller's tag from alto_server_submit */
+    int db;                /* database (engine index) */
+    int status;            /* 0 answered; 1 wrong length; 2, 3 as alto_answer; 9 no free slot;
+                              -1 engine failure */
+    int slot;              /* response slot to release (status 0), else -1 */
+    const uint8_t *response;
+    size_t response_len;
+    int64_t t_submit_ns;   /* steady-clock time of submit and of completion */
+    int64_t t_done_ns;
+    int batch;             /* size of the batch that answered it (0 if never batched) */
+} alto_completion;
+
+typedef struct {
+    long long submitted, queue_full, batches, answered_in_batches, max_batch_seen;
+} alto_server_stats;
+
+void *alto_server_new(void *const *engines, int n_engines, int max_batch, int slots_per_db,
+                      int contexts_per_db);
+/* Returns 0 (queued), 1 (wrong length) or 9 (no free slot; both completed at once with that
+ * status), -1 on bad arguments. */
+int alto_server_submit(void *srv, int db, const uint8_t *frame, size_t len, uint64_t tag);
+/* Waits up to timeout_ms for completions; returns how many were written to out. */
+int alto_server_poll(void *srv, alto_completion *out, int max, int timeout_ms);
+int alto_server_release(void *srv, int db, int slot);
+int alto_server_contexts(void *srv, int db);
+int alto_server_get_stats(void *srv, alto_server_stats *out);
+const char *alto_server_error(void *srv);
+void alto_server_free(void *srv);
 
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */

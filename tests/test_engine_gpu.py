@@ -197,3 +197,44 @@ def test_every_kernel_variant_gives_the_same_bytes(hip_device, B):
         bad = [c for c in range(B) if not np.array_equal(out[c], out_o[c])]
         assert not bad, f"variant {name}: responses {bad[:8]} differ"
     eng.close()
+
+
+def test_graph_replays_match_eager_batches(hip_device, small_inputs):
+    # The device entry point captures a batch into a HIP graph and replays it for the same
+    # pointers; every replay must equal an eager run of the same frames, with other eager
+    # work and synchronisations between replays.
+    import torch
+
+    inp = small_inputs
+    B = 6
+    L, rl = inp.request_len, inp.response_len
+    stride = (7 + L + 255) // 256 * 256
+    eng = Engine.from_inputs(inp)
+    slots = torch.zeros((B, stride), dtype=torch.uint8, device="cuda")
+    status = torch.zeros(B, dtype=torch.int32, device="cuda")
+    out = torch.zeros((B, rl), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    for rep in range(4):
+        reqs = workload.make_requests(B, inp.rows, inp.kp, seed=700 + rep)
+        if rep == 2:
+            reqs[3] = bytes(reqs[3][:9]) + b"\xff\xff\xff\xff" + bytes(reqs[3][13:])  # key word >= Q
+        host = np.zeros((B, stride), np.uint8)
+        for c, f in enumerate(reqs):
+            host[c, 7:7 + L] = np.frombuffer(bytes(f), np.uint8)
+        slots.copy_(torch.from_numpy(host))
+        status.zero_()
+        out.zero_()
+        torch.cuda.synchronize()
+        eng.set_graphs(True)
+        assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(),
+                                 stream.cuda_stream) == 0
+        stream.synchronize()
+        got, st = out.cpu().numpy(), status.cpu().numpy()
+        _ = torch.ones(1 << 20, device="cuda").sum().item()  # unrelated eager work
+        out_e = np.zeros((B, rl), np.uint8)
+        st_e = np.full(B, -1, np.int32)
+        eng.answer(reqs, out_e, st_e)
+        assert np.array_equal(st, st_e), f"replay {rep}: statuses"
+        ok = st_e == 0
+        assert np.array_equal(got[ok], out_e[ok]), f"replay {rep}: responses differ from the eager batch"
+    eng.close()

@@ -1,0 +1,114 @@
+"""The device-resident serving loop: every response equals the oracle's, alone and in a mix
+of concurrent requests (including malformed ones) answered in shared batches."""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+from altopir_public_amd import workload
+from altopir_public_amd.engine import Engine
+from altopir_public_amd.serve import Server, run_load
+import oracle_ctypes as oc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def two_dbs():
+    a = workload.make_inputs(2048, 4096, n=2048, tg=4, seed=11)
+    b = workload.make_inputs(4096, 2048, n=2048, tg=4, seed=12, prefix="random")
+    return a, b
+
+
+def oracle_responses(inp, frames):
+    B = len(frames)
+    out = np.zeros((B, inp.response_len), np.uint8)
+    st = np.full(B, -1, np.int32)
+    oc.oracle_answer(inp, frames, out, st)
+    return out, st
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs):
+    inps = list(two_dbs)
+    engines = [Engine.from_inputs(i) for i in inps]
+    srv = Server(engines, max_batch=16, slots=64, contexts=2)
+    rng = np.random.default_rng(5)
+    frames, dbs = [], []
+    for k in range(40):
+        d = int(rng.integers(2))
+        inp = inps[d]
+        f = workload.make_requests(1, inp.rows, inp.kp, seed=900 + k)[0]
+        if k % 7 == 3:
+            f = workload.corrupt(f, rng, inp.kp)
+        frames.append(bytes(f))
+        dbs.append(d)
+    want_status = [workload.frame_status(f, inps[d].rows, inps[d].kp) for f, d in zip(frames, dbs)]
+    want = {}
+    for d in range(2):
+        idx = [i for i in range(len(frames)) if dbs[i] == d]
+        out, st = oracle_responses(inps[d], [frames[i] for i in idx])
+        for j, i in enumerate(idx):
+            assert st[j] == want_status[i]
+            want[i] = out[j].tobytes() if st[j] == 0 else None
+
+    # alone: one request in the server at a time
+    alone = {}
+    for i, (f, d) in enumerate(zip(frames, dbs)):
+        assert srv.submit(d, f, i) in (0, 1)
+        got = []
+        while not got:
+            got = srv.poll(100)
+        assert len(got) == 1 and got[0].tag == i and got[0].batch <= 1
+        alone[i] = got[0]
+
+    # the mix: two threads submit everything at once
+    mix = {}
+
+    def sender(part):
+        for i in part:
+            assert srv.submit(dbs[i], frames[i], i) in (0, 1)
+
+    ts = [threading.Thread(target=sender, args=(list(range(p, len(frames), 2)),)) for p in range(2)]
+    for t in ts:
+        t.start()
+    while len(mix) < len(frames):
+        for x in srv.poll(100):
+            mix[x.tag] = x
+    for t in ts:
+        t.join()
+    assert max(x.batch for x in mix.values()) > 1, "the mix never shared a batch"
+    for i in range(len(frames)):
+        assert alone[i].status == mix[i].status == want_status[i], i
+        if want_status[i] == 0:
+            assert alone[i].response == want[i], f"request {i}: alone differs from the oracle"
+            assert mix[i].response == want[i], f"request {i}: in the mix differs from the oracle"
+    srv.close()
+    for e in engines:
+        e.close()
+
+
+def test_poisson_load_reports_latency_and_rejects_corrupt_frames(hip_device, two_dbs):
+    inp = two_dbs[0]
+    eng = Engine.from_inputs(inp)
+    srv = Server([eng], max_batch=32, slots=128, contexts=2)
+    rng = np.random.default_rng(9)
+    base = workload.make_requests(8, inp.rows, inp.kp, seed=77)
+    frames = [bytes(base[i % 8]) if i % 25 else workload.corrupt(base[i % 8], rng, inp.kp) for i in range(200)]
+    want_status = [workload.frame_status(f, inp.rows, inp.kp) for f in frames]
+    out_o, _ = oracle_responses(inp, base)
+    seen = {}
+
+    def check(i, d):
+        seen[i] = d.status
+        if d.status == 0:
+            assert d.response == out_o[i % 8].tobytes()
+
+    res = run_load(srv, frames, [0] * len(frames), rate=2000.0, seed=5, on_done=check)
+    assert [seen[i] for i in range(len(frames))] == want_status
+    s = res.summary()
+    assert s["answered"] == sum(1 for x in want_status if x == 0)
+    assert s["p99_ms"] >= s["p50_ms"] > 0
+    srv.close()
+    eng.close()
