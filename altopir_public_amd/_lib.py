@@ -18,6 +18,7 @@ SYMBOLS = (
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
     "alto_engine_new_device", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
     "alto_answer_gather", "alto_context_timings",
+    "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
     "alto_server_get_stats", "alto_server_error", "alto_server_free",
@@ -90,6 +91,11 @@ def load() -> C.CDLL:
     lib.alto_answer_gather.argtypes = [P, P, SZ, P, SZ, P, SZ, P, P, P]
     lib.alto_context_timings.restype = C.c_int
     lib.alto_context_timings.argtypes = [P, P, C.c_int]
+    for name in ("matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8"):
+        getattr(lib, name).restype = None
+        getattr(lib, name).argtypes = [P, P, P, SZ, SZ]
+    lib.alto_matmul_vec_packed.restype = C.c_int
+    lib.alto_matmul_vec_packed.argtypes = [P, P, P, SZ, SZ, SZ]
     lib.alto_set_graphs.restype = C.c_int
     lib.alto_set_graphs.argtypes = [P, C.c_int]
     _lib = lib

@@ -109,10 +109,10 @@ void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     else g.gz = 1;
     const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
     if (g.gz > 1)
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(256), lds, st,
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds, st,
                            (const i32x4 *)a, (const i32x4 *)b, g);
     else
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false>), dim3(grid), dim3(256), lds, st,
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds, st,
                            (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
@@ -138,15 +138,34 @@ const Variant kVariants[] = {
      prepare_dma<8, 8, 2, 2, 3, false>, prepare_dma<8, 8, 2, 2, 3, true>},
     {"dma_m8_n8_s4", launch_dma<8, 8, 2, 2, 4, false>, launch_dma<8, 8, 2, 2, 4, true>,
      prepare_dma<8, 8, 2, 2, 4, false>, prepare_dma<8, 8, 2, 2, 4, true>},
+    {"dma8_m4_n8_s3", launch_dma<4, 8, 4, 2, 3, false>, launch_dma<4, 8, 4, 2, 3, true>,
+     prepare_dma<4, 8, 4, 2, 3, false>, prepare_dma<4, 8, 4, 2, 3, true>},
+    {"dma8_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<4, 8, 4, 2, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
+    {"dma8_m8_n4_s3", launch_dma<8, 4, 2, 4, 3, false>, launch_dma<8, 4, 2, 4, 3, true>,
+     prepare_dma<8, 4, 2, 4, 3, false>, prepare_dma<8, 4, 2, 4, 3, true>},
+    {"dma8_m4_n4_s4", launch_dma<4, 4, 4, 2, 4, false>, launch_dma<4, 4, 4, 2, 4, true>,
+     prepare_dma<4, 4, 4, 2, 4, false>, prepare_dma<4, 4, 4, 2, 4, true>},
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
-// Built-in shape policy (used until the tuner records a choice for a batch-size bucket).
-int default_variant(int MT) {
-    if (MT == 1) return 0;
-    if (MT == 2) return 1;
-    if (MT < 16) return 5;
-    return 6;
+// Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
+// from a sweep over batch sizes 1-512 on a 1 GiB database: streaming kernels up to 8
+// queries, the 4-wave ring for 9-16 queries (scan), the 8-wave ring above.
+int variant_index(const char *name);
+int default_variant(int which, int MT) {
+    static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
+                     stream4 = variant_index("stream_m4_n4_u2"), dma4 = variant_index("dma_m4_n8_s4"),
+                     dma8 = variant_index("dma8_m4_n8_s4");
+    if (MT == 1) return stream1;
+    if (MT == 2) return stream2;
+    if (MT <= 4) return which == 0 ? dma4 : stream4;
+    return dma8;
+}
+int variant_index(const char *name) {
+    for (int i = 0; i < kNumVariants; i++)
+        if (!strcmp(kVariants[i].name, name)) return i;
+    return 0;
 }
 
 // Batch-size buckets of the tuning table: A tiles (4 queries each) 1, 2, 3-4, 5-8, ...,
@@ -189,7 +208,7 @@ struct Engine {
     ~Engine();
     int variant_for(int which, int MT) const {
         const int v = tuned[which][bucket_of(MT)];
-        return v >= 0 ? v : default_variant(MT);
+        return v >= 0 ? v : default_variant(which, MT);
     }
     size_t request_len() const { return 9 + 4 * (2 * kp + rows); }
     size_t response_len() const { return outputs * part_len; }
@@ -418,6 +437,72 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     return HIP_OK(hipGraphLaunch(exec, st)) ? 0 : -1;
 }
 
+// out[i * bc + c] += acc[c][i] + 128 * sum(b_c)   (the scan's offset correction, mod 2^32)
+__global__ void packed_finish_kernel(uint32_t *out, const uint32_t *acc, const uint32_t *qsum, size_t rows,
+                                     size_t bc, size_t C) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * bc) return;
+    const size_t i = t / bc, c = t % bc;
+    out[t] += acc[c * C + i] + (qsum[c] << 7);
+}
+
+// The reference's SimplePIR packed product (src/matmul.cpp:105-218, 220-388, 391-674,
+// 1548-1559): out[i * bc + c] += sum_j sum_t byte_t(a[i * aCols + j]) * b_c[4 j + t] (mod
+// 2^32), b_c = b + c * 4 * aCols.  This is the scan of this engine with the bytes of row i
+// of `a` as database column i and the columns of b as queries, so it runs on the same
+// limb preparation and contraction kernels.  Host buffers in and out, one call at a time.
+int packed_product(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t rows, size_t cols, size_t bc) {
+    if (!out || !a || !b || !rows || !cols || !bc) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return -1;
+    }
+    const size_t P = round_up(4 * cols, 64), R16 = round_up(rows, 16), KC = P / 64, CT = R16 / 16;
+    const size_t Bp = round_up(bc, 4), MT = Bp / 4;
+    const size_t S = round_up(16 + 4 * P, 256);
+    uint8_t *stage = nullptr, *slots = nullptr;
+    int8_t *db = nullptr, *qa = nullptr;
+    uint32_t *acc = nullptr, *qsum = nullptr, *dout = nullptr;
+    unsigned long long *ysum = nullptr;
+    int *status = nullptr;
+    hipStream_t st = nullptr;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && HIP_OK(hipMalloc(&stage, R16 * P)) &&
+              HIP_OK(hipMalloc(&db, R16 * P)) && HIP_OK(hipMalloc(&slots, bc * S)) &&
+              HIP_OK(hipMalloc(&qa, Bp * 4 * P)) && HIP_OK(hipMalloc(&acc, Bp * R16 * 4)) &&
+              HIP_OK(hipMalloc(&qsum, Bp * 4)) && HIP_OK(hipMalloc(&ysum, Bp * 8)) &&
+              HIP_OK(hipMalloc(&status, Bp * 4)) && HIP_OK(hipMalloc(&dout, rows * bc * 4));
+    ok = ok && HIP_OK(hipMemsetAsync(stage, 0, R16 * P, st)) && HIP_OK(hipMemsetAsync(slots, 0, bc * S, st)) &&
+         HIP_OK(hipMemsetAsync(qsum, 0, Bp * 4, st)) && HIP_OK(hipMemsetAsync(status, 0, Bp * 4, st)) &&
+         HIP_OK(hipMemsetAsync(acc, 0, Bp * R16 * 4, st)) &&
+         HIP_OK(hipMemcpy2DAsync(stage, P, a, 4 * cols, 4 * cols, rows, hipMemcpyHostToDevice, st)) &&
+         HIP_OK(hipMemcpy2DAsync(slots + 16, S, b, 16 * cols, 16 * cols, bc, hipMemcpyHostToDevice, st)) &&
+         HIP_OK(hipMemcpyAsync(dout, out, rows * bc * 4, hipMemcpyHostToDevice, st));
+    if (ok) {
+        const size_t work = CT * KC * 64;
+        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, (const uint8_t *)stage,
+                           P, db, CT, KC, (size_t)0);
+        hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
+                           st, FrameSet{slots, S, nullptr}, (size_t)0, (const int *)status, (int)bc, KC, qa, qsum, ysum);
+        GemmArgs g{};
+        g.KC = (int)KC;
+        g.MT = (int)MT;
+        g.NT = (int)CT;
+        g.C = (int)R16;
+        g.out = acc;
+        kVariants[default_variant(0, (int)MT)].scan(g, qa, db, st);
+        hipLaunchKernelGGL(packed_finish_kernel, dim3((unsigned)((rows * bc + 255) / 256)), dim3(256), 0, st, dout,
+                           (const uint32_t *)acc, (const uint32_t *)qsum, rows, bc, R16);
+        ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(out, dout, rows * bc * 4, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipStreamSynchronize(st));
+    }
+    for (void *p : {(void *)stage, (void *)db, (void *)slots, (void *)qa, (void *)acc, (void *)qsum, (void *)ysum,
+                    (void *)status, (void *)dout})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
 bool tail_config_ok(size_t n, uint64_t q1, int q1_bits, size_t row1_off, size_t part_len) {
     // sw_intt_q (src/amx_kernels.cpp:463) and sw_switch_row1 (src/amx_kernels.cpp:517)
     if (n < 16 || (n & (n - 1))) return false;
@@ -620,6 +705,23 @@ void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t row
 }
 
 void alto_engine_free(void *p) { delete (Engine *)p; }
+
+void matMulVecPacked(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t aRows, size_t aCols) {
+    if (packed_product(out, a, b, aRows, aCols, 1)) fprintf(stderr, "matMulVecPacked: %s\n", g_err.c_str());
+}
+void matMulVecPacked2(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols) {
+    if (packed_product(out, a, b_full, aRows, aCols, 2)) fprintf(stderr, "matMulVecPacked2: %s\n", g_err.c_str());
+}
+void matMulVecPacked4(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols) {
+    if (packed_product(out, a, b_full, aRows, aCols, 4)) fprintf(stderr, "matMulVecPacked4: %s\n", g_err.c_str());
+}
+void matMulVecPacked8(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols) {
+    if (packed_product(out, a, b_full, aRows, aCols, 8)) fprintf(stderr, "matMulVecPacked8: %s\n", g_err.c_str());
+}
+int alto_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t a_rows, size_t a_cols,
+                           size_t b_cols) {
+    return packed_product(out, a, b, a_rows, a_cols, b_cols);
+}
 
 size_t alto_request_len(const void *p) { return ((const Engine *)p)->request_len(); }
 size_t alto_response_len(const void *p) { return ((const Engine *)p)->response_len(); }

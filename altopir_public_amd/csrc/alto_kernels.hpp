@@ -508,20 +508,21 @@ __device__ __forceinline__ void stage_barrier() {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Multi-stage contraction for many queries (matrix-core bound).  Like lds_kernel, but the
-// operand blocks travel from memory straight into LDS (global_load_lds_dwordx4: lane l's
+// Multi-stage contraction for many queries (matrix-core bound).  Like lds_kernel (with 4 or
+// 8 waves), but the operand blocks travel from memory straight into LDS (global_load_lds_dwordx4: lane l's
 // 16 bytes land at 16 * l of the destination, which is exactly the block layout) through a
 // ring of S stages, so S - 1 k-steps of loads are in flight while the matrix cores work
 // and no VGPRs hold prefetched data.  Needs S * (BM + BN) KiB of dynamic LDS.
 template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC>
-__global__ __launch_bounds__(256, 1) void dma_kernel(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
-                                                     GemmArgs g) {
-    static_assert(WM * WN == 4, "four waves");
+__global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__restrict__ a_src,
+                                                              const i32x4 *__restrict__ b_src, GemmArgs g) {
+    constexpr int NW = WM * WN;  // waves: 4 (one per SIMD) or 8 (two per SIMD)
+    static_assert(NW == 4 || NW == 8, "four or eight waves");
     static_assert(S >= 2 && S <= 4, "stages");
     constexpr int BM = WM * TM, BN = WN * TN, NBLK = BM + BN;
-    static_assert(NBLK % 4 == 0, "blocks split evenly over the waves");
+    static_assert(NBLK % NW == 0, "blocks split evenly over the waves");
     static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
-    constexpr int PW = NBLK / 4;
+    constexpr int PW = NBLK / NW;
     constexpr int BS = b_kstride<PACK>();
     extern __shared__ i32x4 ring[];  // [S][NBLK][64]
     const TileMap tm = map_tile(g.gx, g.gy, g.gz);
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(256, 1) void dma_kernel(const i32x4 *__restrict__ a
     int kst[PW];
 #pragma unroll
     for (int p = 0; p < PW; p++) {
-        const int blk = wave + 4 * p;
+        const int blk = wave + NW * p;
         if (blk < BM) {
             src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
             kst[p] = 64;
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(256, 1) void dma_kernel(const i32x4 *__restrict__ a
 #pragma unroll
         for (int p = 0; p < PW; p++)
             __builtin_amdgcn_global_load_lds((const void *)(src[p] + (size_t)k * kst[p]),
-                                             (lds_void *)(dst + (wave + 4 * p) * 64), 16, 0, 0);
+                                             (lds_void *)(dst + (wave + NW * p) * 64), 16, 0, 0);
     };
     i32x4 acc[TM][TN];
 #pragma unroll

@@ -80,7 +80,22 @@ size_t alto_response_len(const void *e);
 int alto_answer(void *e, const uint8_t *const *req, const size_t *len, size_t B,
                 uint8_t *const *out, int *status, int method, double *stage_ms);
 
+/* replace matMulVecPacked / matMulVecPacked2 / 4 / 8, src/matmul.cpp:105, 220, 676, 1548
+ * (declared at src/matmul.rs:3-26), the SimplePIR packed product off the engine path:
+ * out[i * bc + c] += sum_j sum_{t<4} byte_t(a[i * aCols + j]) * b[c * 4 * aCols + 4 j + t]
+ * (mod 2^32) for bc = 1, 2, 4, 8 columns of b.  Host buffers; runs on the scan kernels;
+ * exactly aRows rows are written (the reference works in groups of eight rows). */
+void matMulVecPacked(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t aRows, size_t aCols);
+void matMulVecPacked2(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols);
+void matMulVecPacked4(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols);
+void matMulVecPacked8(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols);
+
 /* ---------------------------------------------------------------- Extensions -- */
+
+/* The packed product for any number of columns of b; 0, or -1 (bad arguments, no device,
+ * device memory). */
+int alto_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t a_rows, size_t a_cols,
+                           size_t b_cols);
 
 /* Device-resident batch.  Frame c lies at slots + c * slot_stride + ALTO_FRAME_OFFSET in
  * device memory (slots 16-byte aligned, slot_stride a multiple of 16 and at least

@@ -369,3 +369,21 @@ uint64_t orc_count_rescale_mismatch(uint64_t lo, uint64_t hi, uint64_t q1) {
     for (uint64_t v = lo; v < hi; v++) bad += orc_rescale_q1(v, q1) != orc_rescale_i64(v, ORC_Q, q1);
     return bad;
 }
+
+/* SimplePIR packed product, src/matmul.cpp:105-218 (bc = 1) and its 2/4/8-column forms
+ * (:220-388, :391-674, :1548-1559): b column c starts at b + c * 4 * cols, and
+ * out[i * bc + c] += sum_j sum_{t<4} byte_t(a[i * cols + j]) * b_c[4 j + t]  (mod 2^32). */
+void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t rows, size_t cols,
+                           size_t bc) {
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < rows; i++)
+        for (size_t c = 0; c < bc; c++) {
+            const uint32_t *bcol = b + c * 4 * cols;
+            uint32_t s = 0;
+            for (size_t j = 0; j < cols; j++) {
+                const uint32_t w = a[i * cols + j];
+                for (size_t t = 0; t < 4; t++) s += ((w >> (8 * t)) & 0xffu) * bcol[4 * j + t];
+            }
+            out[i * bc + c] += s;
+        }
+}

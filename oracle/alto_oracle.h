@@ -25,6 +25,7 @@
  *   orc_write_bits    the golden bit patterns of src/bits.rs:115-133
  *   orc_pack_build    compiled reference (sw_pack_build)
  *   orc_answer        compiled reference, byte for byte, on seeded inputs
+ *   orc_matmul_vec_packed  compiled reference (src/matmul.cpp) and brute force
  */
 #ifndef ALTO_ORACLE_H
 #define ALTO_ORACLE_H
@@ -82,6 +83,9 @@ void orc_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
                     const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg,
                     size_t n, size_t rho);
 void orc_tile_db(uint8_t *dst, const uint8_t *db, size_t ld, size_t rows, size_t cols);
+
+void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t rows, size_t cols,
+                           size_t bc);
 
 /* Forward negacyclic NTT under the same slot convention, used only to pin orc_intt. */
 void orc_ntt_naive(uint64_t *out, const uint64_t *in, size_t n, uint64_t psi);

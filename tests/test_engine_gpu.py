@@ -238,3 +238,22 @@ def test_graph_replays_match_eager_batches(hip_device, small_inputs):
         ok = st_e == 0
         assert np.array_equal(got[ok], out_e[ok]), f"replay {rep}: responses differ from the eager batch"
     eng.close()
+
+
+@pytest.mark.parametrize("rows,cols", [(64, 96), (50, 37), (1024, 512)])
+def test_packed_product_matches_the_oracle(hip_device, rows, cols):
+    # the SimplePIR packed product of src/matmul.cpp, served by the scan kernels
+    import ctypes as C
+
+    from altopir_public_amd import _lib, synthetic
+
+    lib, orc = _lib.load(), oc.oracle()
+    orc.orc_matmul_vec_packed.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 3
+    for bc, fn in ((1, "matMulVecPacked"), (2, "matMulVecPacked2"), (4, "matMulVecPacked4"), (8, "matMulVecPacked8")):
+        a = synthetic.uniform_u32(rows + bc, rows * cols)
+        b = synthetic.uniform_u32(cols + bc, bc * 4 * cols)
+        out = synthetic.uniform_u32(bc, rows * bc + 8)
+        want, got = out.copy(), out.copy()
+        orc.orc_matmul_vec_packed(want.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols, bc)
+        getattr(lib, fn)(got.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols)
+        assert np.array_equal(got, want), fn

@@ -322,3 +322,44 @@ def test_pack_build_matches_the_compiled_reference():
     rlib.sw_pack_build(ub_r.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
                        num_iter, tg, n, rho, Q, 4)
     assert np.array_equal(ub_o, ub_r)
+
+
+def _matmul_inputs(rows, cols, bc, seed):
+    a = synthetic.uniform_u32(seed, rows * cols)
+    b = synthetic.uniform_u32(seed + 1, bc * 4 * cols)
+    out = synthetic.uniform_u32(seed + 2, rows * bc + 8)
+    return a, b, out
+
+
+def test_matmul_vec_packed_brute_force():
+    lib = oc.oracle()
+    lib.orc_matmul_vec_packed.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 3
+    rows, cols = 16, 5
+    for bc in (1, 2, 4, 8):
+        a, b, out = _matmul_inputs(rows, cols, bc, 40 + bc)
+        got = out.copy()
+        lib.orc_matmul_vec_packed(got.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols, bc)
+        for i in range(rows):
+            for c in range(bc):
+                s = int(out[i * bc + c])
+                for j in range(cols):
+                    w = int(a[i * cols + j])
+                    for t in range(4):
+                        s += ((w >> (8 * t)) & 0xFF) * int(b[c * 4 * cols + 4 * j + t])
+                assert int(got[i * bc + c]) == s % (1 << 32)
+
+
+@pytest.mark.skipif(not (oc.ORACLE_DIR / "_ref" / "libmatmul_ref.so").exists(), reason="oracle/_ref not built")
+def test_matmul_vec_packed_matches_the_compiled_reference():
+    lib = oc.oracle()
+    lib.orc_matmul_vec_packed.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 3
+    ref = C.CDLL(str(oc.ORACLE_DIR / "_ref" / "libmatmul_ref.so"))
+    rows, cols = 64, 96
+    for bc, fn in ((1, "matMulVecPacked"), (2, "matMulVecPacked2"), (4, "matMulVecPacked4"), (8, "matMulVecPacked8")):
+        a, b, out = _matmul_inputs(rows, cols, bc, 60 + bc)
+        o1, o2 = out.copy(), out.copy()
+        lib.orc_matmul_vec_packed(o1.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols, bc)
+        f = getattr(ref, fn)
+        f.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 2
+        f(o2.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols)
+        assert np.array_equal(o1, o2), fn
