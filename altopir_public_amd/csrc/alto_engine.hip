@@ -116,6 +116,31 @@ void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
                            (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
+template <int TM, int TN, int WM, int WN, int S, bool PACK>
+void prepare_dma2() {
+    constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
+    (void)hipFuncSetAttribute((const void *)dma2_kernel<TM, TN, WM, WN, S, PACK, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)dma2_kernel<TM, TN, WM, WN, S, PACK, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <int TM, int TN, int WM, int WN, int S, bool PACK>
+void launch_dma2(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
+    g.gx = ceil_div(g.NT, WN * TN);
+    g.gy = ceil_div(g.MT, WM * TM);
+    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
+    else g.gz = 1;
+    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    if (g.gz > 1)
+        hipLaunchKernelGGL((dma2_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds, st,
+                           (const i32x4 *)a, (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((dma2_kernel<TM, TN, WM, WN, S, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds, st,
+                           (const i32x4 *)a, (const i32x4 *)b, g);
+}
+
 // Kernel variants of a contraction.  Every variant computes the full result for any number
 // of A tiles (ragged tiles are clamped and masked); they differ in tile shapes and in how
 // operands are staged, which the tuner chooses per batch size.
@@ -146,6 +171,14 @@ const Variant kVariants[] = {
      prepare_dma<8, 4, 2, 4, 3, false>, prepare_dma<8, 4, 2, 4, 3, true>},
     {"dma8_m4_n4_s4", launch_dma<4, 4, 4, 2, 4, false>, launch_dma<4, 4, 4, 2, 4, true>,
      prepare_dma<4, 4, 4, 2, 4, false>, prepare_dma<4, 4, 4, 2, 4, true>},
+    {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
+     prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
+    {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,
+     prepare_dma2<4, 8, 4, 2, 3, false>, prepare_dma2<4, 8, 4, 2, 3, true>},
+    {"dma2_4w_m8_n8_s4", launch_dma2<8, 8, 2, 2, 4, false>, launch_dma2<8, 8, 2, 2, 4, true>,
+     prepare_dma2<8, 8, 2, 2, 4, false>, prepare_dma2<8, 8, 2, 2, 4, true>},
+    {"dma2_8w_m8_n4_s4", launch_dma2<8, 4, 2, 4, 4, false>, launch_dma2<8, 4, 2, 4, 4, true>,
+     prepare_dma2<8, 4, 2, 4, 4, false>, prepare_dma2<8, 4, 2, 4, 4, true>},
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 

@@ -598,6 +598,113 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__res
     }
 }
 
+// dma_kernel with the LDS reads of a k-step overlapped with the matrix work of the step
+// before: the barrier that publishes stage k+1 is taken before the MFMAs of stage k, the
+// fragments of stage k+1 are read into a second register set while those MFMAs run, and
+// the loop alternates between the two sets.  Stage k+S-1 is then issued into the buffer of
+// stage k-1, whose fragments were consumed before that barrier.  Needs S >= 3.
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__restrict__ a_src,
+                                                               const i32x4 *__restrict__ b_src, GemmArgs g) {
+    constexpr int NW = WM * WN;
+    static_assert(NW == 4 || NW == 8, "four or eight waves");
+    static_assert(S >= 3 && S <= 4, "stages");
+    constexpr int BM = WM * TM, BN = WN * TN, NBLK = BM + BN;
+    static_assert(NBLK % NW == 0, "blocks split evenly over the waves");
+    static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    constexpr int PW = NBLK / NW;
+    constexpr int BS = b_kstride<PACK>();
+    extern __shared__ i32x4 ring[];  // [S][NBLK][64]
+    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave / WN, wn = wave % WN;
+    const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    const i32x4 *src[PW];
+    int kst[PW];
+#pragma unroll
+    for (int p = 0; p < PW; p++) {
+        const int blk = wave + NW * p;
+        if (blk < BM) {
+            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            kst[p] = 64;
+        } else {
+            src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
+            kst[p] = BS;
+        }
+    }
+    auto issue = [&](int k) {
+        i32x4 *dst = ring + (size_t)(((k - kb) % S) * NBLK) * 64;
+#pragma unroll
+        for (int p = 0; p < PW; p++)
+            __builtin_amdgcn_global_load_lds((const void *)(src[p] + (size_t)k * kst[p]),
+                                             (lds_void *)(dst + (wave + NW * p) * 64), 16, 0, 0);
+    };
+    // waits until stage k is in LDS for every wave (`later` = stages issued after k)
+    auto publish = [&](int later) {
+        if (later >= 2) wait_vmcnt<2 * PW>();
+        else if (later == 1) wait_vmcnt<PW>();
+        else wait_vmcnt<0>();
+        stage_barrier();
+    };
+    i32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+    i32x4 ca[TM], cb[TN], na[TM], nb[TN];
+#define ALTO_READ(K, FA, FB)                                                             \
+    {                                                                                    \
+        const i32x4 *cur = ring + (size_t)((((K) - kb) % S) * NBLK) * 64;                \
+        _Pragma("unroll") for (int n = 0; n < TN; n++) FB[n] = cur[(BM + wn * TN + n) * 64 + lane]; \
+        _Pragma("unroll") for (int m = 0; m < TM; m++) FA[m] = cur[(wm * TM + m) * 64 + lane];      \
+    }
+#pragma unroll
+    for (int s = 0; s < S - 1; s++)
+        if (kb + s < ke) issue(kb + s);
+    publish(min(S - 2, ke - 1 - kb));
+    ALTO_READ(kb, ca, cb)
+    // everything read so far (these fragments, the kernel arguments) is complete before the
+    // loop, so that inside it only the next step's LDS reads are outstanding and the
+    // matrix work on the current fragments does not wait for them
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    for (int k = kb; k < ke; k++) {
+        if (k + 1 < ke) {
+            publish(min(S - 3, ke - 2 - k));  // step k+1 visible; step k-1 fully consumed
+            if (k + S - 1 < ke) issue(k + S - 1);
+            ALTO_READ(k + 1, na, nb)
+        }
+#pragma unroll
+        for (int m = 0; m < TM; m++)
+#pragma unroll
+            for (int n = 0; n < TN; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[m], cb[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < TM; m++) ca[m] = na[m];
+#pragma unroll
+        for (int n = 0; n < TN; n++) cb[n] = nb[n];
+    }
+#undef ALTO_READ
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int rt = mt0 + wm * TM + m;
+        if (rt >= g.MT) continue;
+        if (PACK) {
+#pragma unroll
+            for (int n = 0; n < TN; n += 4) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < TN; n++) {
+                const int t = bt0 + wn * TN + n;
+                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------- tail --
 
 struct TailArgs {

@@ -86,6 +86,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--also-batch", type=int, nargs="*", default=[1],
+                    help="further batch sizes measured on the same engine and reported under 'also'")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -118,13 +120,7 @@ def main() -> int:
     log(f"[bench] engine set up in {time.time() - t0:.1f} s, {eng.device_bytes() / 2**30:.2f} GiB resident")
     L, rl = inp.request_len, inp.response_len
     stride = round_up(7 + L, 256)
-    reqs = workload.make_requests(B, R, inp.kp, seed=2)
-    host = np.zeros((B, stride), np.uint8)
-    for c, f in enumerate(reqs):
-        host[c, 7:7 + L] = f
-    slots = torch.from_numpy(host).to("cuda")
-    status = torch.zeros(B, dtype=torch.int32, device="cuda")
-    out = torch.empty((B, rl), dtype=torch.uint8, device="cuda")
+    KP = tg * n
     torch.cuda.synchronize()
     # a stream of our own (default priority): the engine launches on it and the timing
     # events below are recorded on it
@@ -133,59 +129,73 @@ def main() -> int:
     stream = work.cuda_stream
     assert stream, "expected a non-null stream handle"
 
-    def step():
-        rc = eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(), stream)
-        if rc != 0:
-            raise RuntimeError(f"alto_answer_device returned {rc}")
+    def measure(Bm: int, steps: int, warmup: int, seed: int) -> dict:
+        # request frames generated in device memory (the same bytes as workload.make_requests)
+        slots = workload.make_requests_device(Bm, R, inp.kp, seed=seed, stride=stride)
+        status = torch.zeros(Bm, dtype=torch.int32, device="cuda")
+        out = torch.empty((Bm, rl), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    assert int((status != 0).sum()) == 0, "a valid request was rejected"
-    log("[bench] warm-up done")
+        def step():
+            rc = eng.answer_device(slots.data_ptr(), stride, Bm, out.data_ptr(), rl, status.data_ptr(), stream)
+            if rc != 0:
+                raise RuntimeError(f"alto_answer_device returned {rc}")
 
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(args.steps):
-        step()
-    end.record()
-    torch.cuda.synchronize()
-    ms_per_step = start.elapsed_time(end) / args.steps
-    qps = B / (ms_per_step / 1e3)
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {qps:.1f} QPS")
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        assert int((status != 0).sum()) == 0, "a valid request was rejected"
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(steps):
+            step()
+        end.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / steps
+        # per-kernel device times: HIP events recorded by the engine around each launch
+        # (the engine records them with graph replay off)
+        eng.set_graphs(False)
+        stages = []
+        for _ in range(min(steps, 10)):
+            step()
+            stages.append(eng.last_timings())
+        eng.set_graphs(True)
+        st = np.array(stages).mean(axis=0)
+        names = ["validate", "scan_prep", "scan", "pack_prep", "pack", "tail"]
+        stage_ms = {k: round(float(v), 4) for k, v in zip(names, st)}
+        scan_ops, pack_ops = 8 * R * C * Bm, 32 * KP * C * Bm
+        scan_bytes, pack_bytes = C * R + 4 * Bm * R, 4 * KP * C + 4 * Bm * KP
+        dom = "scan" if stage_ms["scan"] >= stage_ms["pack"] else "pack"
+        t_dom = stage_ms[dom] / 1e3
+        ops, byts = (scan_ops, scan_bytes) if dom == "scan" else (pack_ops, pack_bytes)
+        # ridge: 5.0e15 op/s / 8.0e12 B/s = 625 op/B
+        if ops / byts > 625.0:
+            achieved = ops / t_dom / 1e12
+            roof = {"bound": "mfma", "kernel": f"{dom}_contraction", "achieved": round(achieved, 2),
+                    "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                    "traffic": None, "algorithmic_ops": ops, "algorithmic_bytes": byts}
+        else:
+            achieved = byts / t_dom / 1e9
+            roof = {"bound": "hbm", "kernel": f"{dom}_contraction", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None, "algorithmic_bytes": byts, "algorithmic_ops": ops}
+        res = {"batch": Bm, "qps": round(Bm / (ms / 1e3), 2), "ms_per_step": round(ms, 4), "stage_ms": stage_ms,
+               "scan_GBps": round(scan_bytes / (stage_ms["scan"] / 1e3) / 1e9, 1),
+               "scan_TOPs": round(scan_ops / (stage_ms["scan"] / 1e3) / 1e12, 1),
+               "pack_TOPs": round(pack_ops / (stage_ms["pack"] / 1e3) / 1e12, 1), "roofline": roof}
+        log(f"[bench] B={Bm}: {ms:.3f} ms/step, {res['qps']:.1f} QPS, stage ms {stage_ms}")
+        res["_slots"] = slots
+        return res
 
-    # per-kernel device times (HIP events recorded by the engine around each launch; the
-    # engine records them with graph replay off)
-    eng.set_graphs(False)
-    stages = []
-    for _ in range(min(args.steps, 10)):
-        step()
-        stages.append(eng.last_timings())
-    eng.set_graphs(True)
-    st = np.array(stages).mean(axis=0)
-    names = ["validate", "scan_prep", "scan", "pack_prep", "pack", "tail"]
-    stage_ms = {k: round(float(v), 4) for k, v in zip(names, st)}
-    log(f"[bench] stage ms {stage_ms}")
-
-    KP = tg * n
-    scan_ops, pack_ops = 8 * R * C * B, 32 * KP * C * B
-    scan_bytes, pack_bytes = C * R + 4 * B * R, 4 * KP * C + 4 * B * KP
-    dom = "scan" if stage_ms["scan"] >= stage_ms["pack"] else "pack"
-    t_dom = stage_ms[dom] / 1e3
-    ops, byts = (scan_ops, scan_bytes) if dom == "scan" else (pack_ops, pack_bytes)
-    # ridge: 5.0e15 op/s / 8.0e12 B/s = 625 op/B
-    ridge_bound = "mfma" if ops / byts > 625.0 else "hbm"
-    if ridge_bound == "mfma":
-        achieved = ops / t_dom / 1e12
-        roof = {"bound": "mfma", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
-                "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
-                "algorithmic_ops": ops}
-    else:
-        achieved = byts / t_dom / 1e9
-        roof = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": byts}
+    main = measure(B, args.steps, args.warmup, seed=2)
+    ms_per_step, qps, stage_ms, roof = main["ms_per_step"], main["qps"], main["stage_ms"], main["roofline"]
+    slots = main.pop("_slots")
+    also = []
+    for b in args.also_batch:
+        r = measure(b, args.steps, args.warmup, seed=2)
+        r.pop("_slots")
+        also.append(r)
 
     if args.copy_calib:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the
@@ -200,9 +210,20 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        if inp.db_col is None:
-            inp.db_col = synthetic.uniform_bytes(0, R * C)  # the same bytes, on the host
-        cpu = cpu_baseline(inp, reqs, args.cpu_threads, args.cpu_budget)
+        reqs = [f[7:7 + L] for f in slots.cpu().numpy()]
+        if R * C <= (2 << 30):
+            if inp.db_col is None:
+                inp.db_col = synthetic.uniform_bytes(0, R * C)  # the same bytes, on the host
+            cpu = cpu_baseline(inp, reqs, args.cpu_threads, args.cpu_budget)
+        else:
+            # bounded sample: the first n columns (one output part) of the same database, the
+            # same requests; scan, packing and tail all scale linearly in the columns
+            part = workload.make_inputs(R, n, n=n, tg=tg, seed=0)
+            cpu = cpu_baseline(part, reqs, args.cpu_threads, args.cpu_budget)
+            scale = C // n
+            cpu["value"] = round(cpu["value"] / scale, 4)
+            cpu["sample"] = (f"{cpu['sample']}; on {n} of the {C} columns (one of {scale} output parts), "
+                             f"QPS divided by {scale}")
         log(f"[bench] cpu baseline {cpu}")
 
     line = {
@@ -225,6 +246,7 @@ def main() -> int:
         },
         "stage_ms": stage_ms,
         "roofline": roof,
+        "also": also,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
