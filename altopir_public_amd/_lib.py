@@ -17,7 +17,7 @@ SYMBOLS = (
     "alto_answer_device", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
     "alto_engine_new_device", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
-    "alto_answer_gather", "alto_context_timings",
+    "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
     "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
@@ -91,6 +91,10 @@ def load() -> C.CDLL:
     lib.alto_answer_gather.argtypes = [P, P, SZ, P, SZ, P, SZ, P, P, P]
     lib.alto_context_timings.restype = C.c_int
     lib.alto_context_timings.argtypes = [P, P, C.c_int]
+    lib.alto_context_stage_totals.restype = C.c_int
+    lib.alto_context_stage_totals.argtypes = [P, P, C.c_int, C.c_int]
+    lib.alto_stage_totals.restype = C.c_int
+    lib.alto_stage_totals.argtypes = [P, P, C.c_int, C.c_int]
     for name in ("matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8"):
         getattr(lib, name).restype = None
         getattr(lib, name).argtypes = [P, P, P, SZ, SZ]

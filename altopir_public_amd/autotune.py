@@ -43,10 +43,7 @@ class DeviceBatch:
         self.L = inp.request_len
         self.rl = inp.response_len
         self.stride = (7 + self.L + 255) // 256 * 256
-        host = np.zeros((B, self.stride), np.uint8)
-        for c, f in enumerate(workload.make_requests(B, inp.rows, inp.kp, seed=seed)):
-            host[c, 7:7 + self.L] = f
-        self.slots = torch.from_numpy(host).to("cuda")
+        self.slots = workload.make_requests_device(B, inp.rows, inp.kp, seed=seed, stride=self.stride)
         self.status = torch.zeros(B, dtype=torch.int32, device="cuda")
         self.out = torch.empty((B, self.rl), dtype=torch.uint8, device="cuda")
         self.engine = engine
@@ -68,7 +65,10 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict
     stage = 2 if which == SCAN else 4  # index in alto_last_timings
     out = {}
     saved = lib.alto_tune_get(engine.ptr, which, batch.B)
+    mt = (batch.B + 3) // 4
     for idx, name in enumerate(variant_names()):
+        if name.startswith("stream") and mt > 16:
+            continue  # register-streaming shapes re-read the database once per 4..32 queries
         lib.alto_tune_set(engine.ptr, which, batch.B, idx)
         ts = []
         for _ in range(reps + 1):

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -266,7 +267,16 @@ struct Context {
     uint8_t *slots = nullptr, *out = nullptr;
     int *status = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[7] = {};  // start, validate, scan prep, scan, pack prep, pack, tail
+    // Timing events of eager batches: a ring of event sets (start, validate, scan prep,
+    // scan, pack prep, pack, tail), so that back-to-back batches are timed without the host
+    // waiting for each one; a set is folded into the running sums before it is reused.
+    static constexpr int kRing = 32;
+    hipEvent_t ring[kRing][7] = {};
+    bool ring_pending[kRing] = {};
+    int ring_head = -1;          // set of the most recent timed batch
+    hipEvent_t *ev = nullptr;    // = ring[ring_head]
+    double stage_sum[6] = {};    // ms, over the folded sets
+    long long stage_count = 0;
     bool timed_valid = false;
     std::vector<int> hstatus;
     // captured batches, replayed when the same batch (pointers, size, stream, kernel
@@ -290,7 +300,35 @@ struct Context {
     bool init(Engine *eng) {
         e = eng;
         bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto &x : ev) ok = ok && HIP_OK(hipEventCreate(&x));
+        for (auto &set : ring)
+            for (auto &x : set) ok = ok && HIP_OK(hipEventCreate(&x));
+        ev = ring[0];
+        return ok;
+    }
+    // Adds the stage times of ring set k (waiting for it) to the running sums.
+    bool fold(int k) {
+        if (!ring_pending[k]) return true;
+        ring_pending[k] = false;
+        if (!HIP_OK(hipEventSynchronize(ring[k][6]))) return false;
+        for (int i = 0; i < 6; i++) {
+            float f = 0;
+            (void)hipEventElapsedTime(&f, ring[k][i], ring[k][i + 1]);
+            stage_sum[i] += f;
+        }
+        stage_count++;
+        return true;
+    }
+    // Moves to the next event set for a timed batch (folding what it held).
+    hipEvent_t *next_set() {
+        ring_head = (ring_head + 1) % kRing;
+        (void)fold(ring_head);
+        ring_pending[ring_head] = true;
+        ev = ring[ring_head];
+        return ev;
+    }
+    bool fold_all() {
+        bool ok = true;
+        for (int k = 0; k < kRing; k++) ok = fold(k) && ok;
         return ok;
     }
     void drop_graphs() {
@@ -312,8 +350,9 @@ struct Context {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
         release_buffers();
-        for (auto &x : ev)
-            if (x) (void)hipEventDestroy(x);
+        for (auto &set : ring)
+            for (auto &x : set)
+                if (x) (void)hipEventDestroy(x);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -363,7 +402,7 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
                      hipStream_t st, bool timed, bool record_start) {
     const Engine &E = *e;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
-    if (timed && record_start) (void)hipEventRecord(ev[0], st);
+    if (timed && record_start) (void)hipEventRecord(next_set()[0], st);
     hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, st_dev, (uint32_t)E.kp, qsum, ysum);
     if (timed) (void)hipEventRecord(ev[1], st);
     // scan: query limbs, then the contraction into acc
@@ -835,7 +874,7 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     for (size_t c = 0; c < B; c++) ctx->hstatus[c] = (!req[c] || len[c] != want) ? 1 : 0;
     hipStream_t st = ctx->stream;
     const FrameSet fs{ctx->slots, S, nullptr};
-    bool ok = HIP_OK(hipEventRecord(ctx->ev[0], st));
+    bool ok = HIP_OK(hipEventRecord(ctx->next_set()[0], st));
     for (size_t c = 0; ok && c < B; c++)
         if (!ctx->hstatus[c])
             ok = HIP_OK(hipMemcpyAsync(ctx->slots + c * S + ALTO_FRAME_OFFSET, req[c], want, hipMemcpyHostToDevice, st));
@@ -900,6 +939,24 @@ int alto_tune_get(const void *p, int which, size_t B) {
     auto *e = (const Engine *)p;
     if (!e || which < 0 || which > 1 || !B) return -1;
     return e->variant_for(which, (int)(round_up(B, 4) / 4));
+}
+
+int alto_context_stage_totals(void *c, double *ms, int count, int reset) {
+    auto *ctx = (Context *)c;
+    if (!ctx || !ms || count <= 0) return -1;
+    if (!ctx->fold_all()) return -1;
+    const int k = std::min(count, 6);
+    for (int i = 0; i < k; i++) ms[i] = ctx->stage_sum[i];
+    const long long n = ctx->stage_count;
+    if (reset) {
+        std::fill(ctx->stage_sum, ctx->stage_sum + 6, 0.0);
+        ctx->stage_count = 0;
+    }
+    return (int)std::min<long long>(n, INT32_MAX);
+}
+
+int alto_stage_totals(void *p, double *ms, int count, int reset) {
+    return p ? alto_context_stage_totals(((Engine *)p)->ctx0, ms, count, reset) : -1;
 }
 
 int alto_context_timings(void *c, double *ms, int count) {

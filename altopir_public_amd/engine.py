@@ -245,6 +245,14 @@ class Engine:
         k = self._lib.alto_last_timings(self.ptr, C.addressof(ms), 6)
         return list(ms)[: max(k, 0)]
 
+    def stage_totals(self, reset: bool = False) -> tuple[int, list]:
+        """(batches, summed stage ms) over the timed batches since the last reset."""
+        ms = (C.c_double * 6)()
+        k = self._lib.alto_stage_totals(self.ptr, C.addressof(ms), 6, int(reset))
+        if k < 0:
+            raise RuntimeError(f"alto_stage_totals failed ({_lib.last_error()})")
+        return k, list(ms)
+
     def close(self):
         if getattr(self, "ptr", None):
             self._lib.alto_engine_free(self.ptr)

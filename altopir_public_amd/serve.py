@@ -82,14 +82,17 @@ class Server:
         a = np.frombuffer(frame, np.uint8) if isinstance(frame, (bytes, bytearray)) else np.ascontiguousarray(frame)
         return self.lib.alto_server_submit(self.ptr, db, a.ctypes.data if len(a) else None, len(a), tag)
 
-    def poll(self, timeout_ms: int = 10) -> list[Done]:
+    def poll(self, timeout_ms: int = 10, copy: bool = True) -> list[Done]:
+        """Completions so far (waits up to timeout_ms for the first).  A valid request's
+        response is copied out (copy=True) and its slot released either way."""
         k = self.lib.alto_server_poll(self.ptr, self._buf, len(self._buf), timeout_ms)
         out = []
         for i in range(max(k, 0)):
             c = self._buf[i]
             resp = None
             if c.status == 0:
-                resp = C.string_at(c.response, c.response_len)
+                if copy:
+                    resp = C.string_at(c.response, c.response_len)
                 self.lib.alto_server_release(self.ptr, c.db, c.slot)
             out.append(Done(c.tag, c.db, c.status, resp, c.t_submit_ns, c.t_done_ns, c.batch))
         return out
@@ -179,3 +182,51 @@ def run_load(server: Server, frames: list, dbs: list, rate: float, seed: int, on
     return LoadResult(sent=n, answered=int(ok.sum()), rejected={k: v for k, v in statuses.items() if k != 0},
                       wall_s=wall, latencies_ms=lat[ok], batches=st["batches"] - stats0["batches"],
                       max_batch=st["max_batch_seen"])
+
+
+def saturate(server: Server, frames: list, db: int = 0, total: int = 1024, threads: int = 8,
+             window: int = 512) -> dict:
+    """Offered load above capacity: `threads` submitters keep up to `window` requests of
+    database `db` outstanding (frames reused round-robin) until `total` are answered.
+    Returns the answered rate over the whole run and the submit-to-completion latencies."""
+    sem = threading.Semaphore(window)
+    lat = np.full(total, np.nan)
+    statuses = {}
+    got = 0
+    done_evt = threading.Event()
+    t_first = [None]
+
+    def collector():
+        nonlocal got
+        while got < total:
+            for d in server.poll(20, copy=False):
+                statuses[d.status] = statuses.get(d.status, 0) + 1
+                lat[d.tag] = (d.t_done_ns - d.t_submit_ns) / 1e6
+                got += 1
+                sem.release()
+        done_evt.set()
+
+    def submitter(k):
+        for i in range(k, total, threads):
+            sem.acquire()
+            server.submit(db, frames[i % len(frames)], i)
+
+    stats0 = server.stats()
+    th = threading.Thread(target=collector, daemon=True)
+    th.start()
+    t0 = now_ns()
+    subs = [threading.Thread(target=submitter, args=(k,), daemon=True) for k in range(threads)]
+    for t in subs:
+        t.start()
+    for t in subs:
+        t.join()
+    done_evt.wait()
+    wall = (now_ns() - t0) / 1e9
+    st = server.stats()
+    ok = statuses.get(0, 0)
+    batches = st["batches"] - stats0["batches"]
+    v = lat[~np.isnan(lat)]
+    return {"qps": round(ok / wall, 2), "answered": ok, "requests": total, "wall_s": round(wall, 4),
+            "statuses": statuses, "batches": batches, "mean_batch": round(total / max(batches, 1), 1),
+            "p50_ms": round(float(np.percentile(v, 50)), 3), "p99_ms": round(float(np.percentile(v, 99)), 3),
+            "threads": threads, "window": window}

@@ -38,6 +38,22 @@ def round_up(x, m):
     return (x + m - 1) // m * m
 
 
+def measured_traffic(kernel: str, records: int, record_bytes: int, batch: int):
+    """HBM bytes per launch of `kernel` on this workload, from the counter summaries kept
+    under profiles/ (lines `traffic: kernel=... records=... record_bytes=... batch=...
+    bytes_per_launch=...`, written from two rocprofv3 --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, scaled by a plain copy of known size); None when none was measured."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if not line.startswith("traffic:"):
+                continue
+            kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
+            if (kv.get("kernel") == kernel and int(kv.get("records", -1)) == records
+                    and int(kv.get("record_bytes", -1)) == record_bytes and int(kv.get("batch", -1)) == batch):
+                return int(kv["bytes_per_launch"])
+    return None
+
+
 def cpu_baseline(inp, reqs, threads: int, budget_s: float) -> dict:
     """The reference engine compiled from its own source (oracle/_ref) on the host cores,
     method VNNI (AVX-512) or AMX, timed like `altopir bench` (src/bin/altopir.rs:741-826):
@@ -88,6 +104,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--also-batch", type=int, nargs="*", default=[1],
                     help="further batch sizes measured on the same engine and reported under 'also'")
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="batches timed through the synchronous host entry point (0: skip)")
+    ap.add_argument("--no-graph-replay", dest="graph_replay", action="store_false",
+                    help="skip the second pass of the timed steps replayed from captured HIP graphs")
+    ap.add_argument("--seed", type=int, default=2, help="request seed")
+    ap.add_argument("--tune", action="store_true",
+                    help="time every tile-shape variant per batch size first and keep the fastest")
+    ap.add_argument("--serve-batches", type=int, default=8,
+                    help="batches' worth of requests pushed through the serving loop (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -146,22 +171,32 @@ def main() -> int:
         torch.cuda.synchronize()
         assert int((status != 0).sum()) == 0, "a valid request was rejected"
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        start.record()
-        for _ in range(steps):
-            step()
-        end.record()
-        torch.cuda.synchronize()
-        ms = start.elapsed_time(end) / steps
-        # per-kernel device times: HIP events recorded by the engine around each launch
-        # (the engine records them with graph replay off)
+
+        def timed(k: int) -> float:
+            torch.cuda.synchronize()
+            start.record()
+            for _ in range(k):
+                step()
+            end.record()
+            torch.cuda.synchronize()
+            return start.elapsed_time(end) / k
+
+        # timed region: eager launches, each kernel bracketed by the engine's HIP events on
+        # the launch stream (a ring of event sets, summed afterwards), so the per-kernel
+        # durations come from the same steps as the throughput
         eng.set_graphs(False)
-        stages = []
-        for _ in range(min(steps, 10)):
-            step()
-            stages.append(eng.last_timings())
+        step()
+        eng.stage_totals(reset=True)
+        ms = timed(steps)
+        nb, sums = eng.stage_totals(reset=True)
+        assert nb == steps, (nb, steps)
+        st = np.array(sums) / nb
+        # the same steps replayed from the captured HIP graph
         eng.set_graphs(True)
-        st = np.array(stages).mean(axis=0)
+        ms_graph = None
+        if args.graph_replay:
+            step()
+            ms_graph = timed(steps)
         names = ["validate", "scan_prep", "scan", "pack_prep", "pack", "tail"]
         stage_ms = {k: round(float(v), 4) for k, v in zip(names, st)}
         scan_ops, pack_ops = 8 * R * C * Bm, 32 * KP * C * Bm
@@ -170,32 +205,77 @@ def main() -> int:
         t_dom = stage_ms[dom] / 1e3
         ops, byts = (scan_ops, scan_bytes) if dom == "scan" else (pack_ops, pack_bytes)
         # ridge: 5.0e15 op/s / 8.0e12 B/s = 625 op/B
+        traffic = measured_traffic(f"{dom}_contraction", R, C, Bm)
         if ops / byts > 625.0:
             achieved = ops / t_dom / 1e12
             roof = {"bound": "mfma", "kernel": f"{dom}_contraction", "achieved": round(achieved, 2),
                     "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                    "traffic": None, "algorithmic_ops": ops, "algorithmic_bytes": byts}
+                    "traffic": traffic, "algorithmic_ops": ops, "algorithmic_bytes": byts}
         else:
             achieved = byts / t_dom / 1e9
             roof = {"bound": "hbm", "kernel": f"{dom}_contraction", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "algorithmic_bytes": byts, "algorithmic_ops": ops}
+                    "traffic": traffic, "algorithmic_bytes": byts, "algorithmic_ops": ops}
         res = {"batch": Bm, "qps": round(Bm / (ms / 1e3), 2), "ms_per_step": round(ms, 4), "stage_ms": stage_ms,
+               "graph_replay": ({"qps": round(Bm / (ms_graph / 1e3), 2), "ms_per_step": round(ms_graph, 4)}
+                                if ms_graph else None),
                "scan_GBps": round(scan_bytes / (stage_ms["scan"] / 1e3) / 1e9, 1),
                "scan_TOPs": round(scan_ops / (stage_ms["scan"] / 1e3) / 1e12, 1),
                "pack_TOPs": round(pack_ops / (stage_ms["pack"] / 1e3) / 1e12, 1), "roofline": roof}
-        log(f"[bench] B={Bm}: {ms:.3f} ms/step, {res['qps']:.1f} QPS, stage ms {stage_ms}")
+        gr = f" ({ms_graph:.3f} graph replay)" if ms_graph else ""
+        log(f"[bench] B={Bm}: {ms:.3f} ms/step{gr}, {res['qps']:.1f} QPS, stage ms {stage_ms}")
         res["_slots"] = slots
         return res
 
-    main = measure(B, args.steps, args.warmup, seed=2)
+    tuned = None
+    if args.tune:
+        from altopir_public_amd import autotune
+
+        eng.set_graphs(False)
+        table = autotune.tune(eng, inp, [B, *args.also_batch], seed=7, log=log)
+        eng.set_graphs(True)
+        tuned = {str(v["B"]): {"scan": v["scan"][0], "pack": v["pack"][0]} for v in table.values()}
+
+    main = measure(B, args.steps, args.warmup, seed=args.seed)
     ms_per_step, qps, stage_ms, roof = main["ms_per_step"], main["qps"], main["stage_ms"], main["roofline"]
     slots = main.pop("_slots")
     also = []
     for b in args.also_batch:
-        r = measure(b, args.steps, args.warmup, seed=2)
+        r = measure(b, args.steps, args.warmup, seed=args.seed)
         r.pop("_slots")
         also.append(r)
+
+    plain = None
+    if args.plain_steps:
+        # the plain form: the synchronous host entry point of the reference's API, frames in
+        # host memory, responses copied back, one batch per call (PCIe transfers included)
+        frames = [np.ascontiguousarray(f[7:7 + L]) for f in slots.cpu().numpy()]
+        hout = np.empty((B, rl), dtype=np.uint8)
+        hst = np.zeros(B, dtype=np.int32)
+        eng.answer(frames, hout, hst)
+        assert int((hst != 0).sum()) == 0
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            eng.answer(frames, hout, hst)
+        dt = (time.perf_counter() - t0) / args.plain_steps
+        plain = {"qps": round(B / dt, 2), "ms_per_step": round(dt * 1e3, 4), "steps": args.plain_steps,
+                 "form": "alto_answer, host frames in pageable memory, responses copied back"}
+        log(f"[bench] plain host form B={B}: {dt * 1e3:.3f} ms/batch, {B / dt:.1f} QPS")
+
+    served = None
+    if args.serve_batches:
+        # the served form: the continuous-batching loop fed from host memory by submitter
+        # threads, uploads overlapped with device work, at an offered load above capacity
+        from altopir_public_amd import serve
+
+        frames = [np.ascontiguousarray(f[7:7 + L]) for f in slots.cpu().numpy()]
+        srv = serve.Server([eng], max_batch=B, slots=3 * B, contexts=2)
+        serve.saturate(srv, frames, total=2 * B, threads=8, window=3 * B)  # warm-up
+        served = serve.saturate(srv, frames, total=args.serve_batches * B, threads=8, window=3 * B)
+        served["form"] = "alto_server_* loop, host frames copied into pinned slots by 8 submitter threads"
+        served["contexts"] = srv.contexts(0)
+        srv.close()
+        log(f"[bench] served form: {served}")
 
     if args.copy_calib:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the
@@ -203,7 +283,9 @@ def main() -> int:
         x = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device="cuda")
         y = torch.empty_like(x)
         for _ in range(3):
-            y.copy_(x)
+            y.copy_(x)  # 4-byte accesses per lane
+        for _ in range(3):
+            y.view(torch.int32).copy_(x.view(torch.int32))  # 16-byte accesses per lane, as the engine's
         torch.cuda.synchronize()
         log(f"[bench] copy calibration: 3 copies of {args.copy_calib} MiB")
 
@@ -241,12 +323,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{R * C / 2**30:g} GiB database ({R} records x {C} B, uniform bytes seed 0), "
-                        f"batch {B} (uniform requests seed 2), requests resident in HBM",
+                        f"batch {B} (uniform requests seed {args.seed}), requests resident in HBM",
             "records": R, "record_bytes": C, "batch": B, "n": n, "tg": tg, "parallelism": "single GPU",
         },
         "stage_ms": stage_ms,
         "roofline": roof,
         "also": also,
+        "tuned": tuned,
+        "plain_host": plain,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -143,6 +143,16 @@ int alto_set_graphs(void *e, int on);
  * `count` values and returns how many were written (0 if no batch ran yet). */
 int alto_last_timings(void *e, double *ms, int count);
 
+/* Running sums of the same six stage times over every timed batch of a context since the
+ * last reset (timed batches: the synchronous entry point, and the device entry points with
+ * graphs off).  Each batch's events are kept in a ring of 32 and folded in when the ring
+ * comes round or here, so back-to-back batches are timed without the host waiting on each.
+ * Waits for the timed batches still in flight; reset != 0 zeroes the sums afterwards.
+ * Returns the number of batches summed, -1 on bad arguments. */
+int alto_context_stage_totals(void *ctx, double *ms, int count, int reset);
+/* alto_context_stage_totals on the engine's own context. */
+int alto_stage_totals(void *e, double *ms, int count, int reset);
+
 /* alto_engine_new with the column-major database already in device memory
  * (db_col_device[col * ld + k]); the engine tiles it into its own copy, after which the
  * caller may free it.  The other arguments and checks are those of alto_engine_new. */

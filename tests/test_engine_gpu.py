@@ -240,6 +240,44 @@ def test_graph_replays_match_eager_batches(hip_device, small_inputs):
     eng.close()
 
 
+def test_stage_totals_count_every_timed_batch(hip_device, small_inputs):
+    # Eager device batches are timed through a ring of 32 event sets; the running totals
+    # must count every one of more than a ring's worth of back-to-back batches, and graph
+    # replays none.
+    import torch
+
+    inp = small_inputs
+    B = 3
+    L, rl = inp.request_len, inp.response_len
+    stride = (7 + L + 255) // 256 * 256
+    eng = Engine.from_inputs(inp)
+    host = np.zeros((B, stride), np.uint8)
+    for c, f in enumerate(workload.make_requests(B, inp.rows, inp.kp, seed=91)):
+        host[c, 7:7 + L] = f
+    slots = torch.from_numpy(host).to("cuda")
+    status = torch.zeros(B, dtype=torch.int32, device="cuda")
+    out = torch.zeros((B, rl), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    eng.set_graphs(False)
+    eng.stage_totals(reset=True)
+    for _ in range(45):
+        assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(),
+                                 stream.cuda_stream) == 0
+    n, ms = eng.stage_totals()
+    assert n == 45 and all(v >= 0 for v in ms) and ms[2] > 0 and ms[5] > 0
+    last = eng.last_timings()
+    assert len(last) == 6 and last[2] > 0
+    eng.set_graphs(True)
+    for _ in range(3):
+        assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(),
+                                 stream.cuda_stream) == 0
+    n2, ms2 = eng.stage_totals(reset=True)
+    assert n2 == 45 and ms2 == ms
+    assert eng.stage_totals()[0] == 0
+    assert int(status.sum()) == 0
+    eng.close()
+
+
 @pytest.mark.parametrize("rows,cols", [(64, 96), (50, 37), (1024, 512)])
 def test_packed_product_matches_the_oracle(hip_device, rows, cols):
     # the SimplePIR packed product of src/matmul.cpp, served by the scan kernels
