@@ -90,17 +90,17 @@ void launch_lds(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
 
 // Allows the dynamic LDS of a dma_kernel instantiation (done once per engine creation,
 // outside any stream capture).
-template <int TM, int TN, int WM, int WN, int S, bool PACK>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1>
 void prepare_dma() {
     constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
-    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true>,
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false>,
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 void prepare_none() {}
 
-template <int TM, int TN, int WM, int WN, int S, bool PACK>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1>
 void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     constexpr int NBLK = WM * TM + WN * TN;
     constexpr size_t lds = (size_t)S * NBLK * 1024;
@@ -110,11 +110,11 @@ void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     else g.gz = 1;
     const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
     if (g.gz > 1)
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds, st,
-                           (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
     else
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds, st,
-                           (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
 template <int TM, int TN, int WM, int WN, int S, bool PACK>
@@ -172,6 +172,18 @@ const Variant kVariants[] = {
      prepare_dma<8, 4, 2, 4, 3, false>, prepare_dma<8, 4, 2, 4, 3, true>},
     {"dma8_m4_n4_s4", launch_dma<4, 4, 4, 2, 4, false>, launch_dma<4, 4, 4, 2, 4, true>,
      prepare_dma<4, 4, 4, 2, 4, false>, prepare_dma<4, 4, 4, 2, 4, true>},
+    {"dma8k2_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false, 2>, launch_dma<4, 8, 4, 2, 4, true, 2>,
+     prepare_dma<4, 8, 4, 2, 4, false, 2>, prepare_dma<4, 8, 4, 2, 4, true, 2>},
+    {"dma8k2_m8_n4_s4", launch_dma<8, 4, 2, 4, 4, false, 2>, launch_dma<8, 4, 2, 4, 4, true, 2>,
+     prepare_dma<8, 4, 2, 4, 4, false, 2>, prepare_dma<8, 4, 2, 4, 4, true, 2>},
+    // 16 x 24-tile workgroups (384 accumulator tiles per CU): 17% fewer operand bytes per
+    // MFMA than 16 x 16; scan only (packing tiles come in fours), packing runs dma8_m4_n8_s4
+    {"dma8_m8_n6_s3", launch_dma<8, 6, 2, 4, 3, false>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<8, 6, 2, 4, 3, false>, prepare_dma<4, 8, 4, 2, 4, true>},
+    {"dma8_m8_n6_s4", launch_dma<8, 6, 2, 4, 4, false>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<8, 6, 2, 4, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
+    {"dma8k2_m8_n6_s4", launch_dma<8, 6, 2, 4, 4, false, 2>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<8, 6, 2, 4, 4, false, 2>, prepare_dma<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
      prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,
@@ -184,17 +196,20 @@ const Variant kVariants[] = {
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
-// from a sweep over batch sizes 1-512 on a 1 GiB database: streaming kernels up to 8
-// queries, the 4-wave ring for 9-16 queries (scan), the 8-wave ring above.
+// from sweeps over batch sizes 1-2048 on 1 and 8 GiB databases: streaming kernels up to 8
+// queries, the 4-wave ring for 9-16 queries (scan), the 8-wave 16 x 16-tile ring up to 128
+// queries and the 16 x 24-tile ring with two k-steps per barrier above (it needs enough
+// query rows to fill the chip with its wider tiles).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), dma4 = variant_index("dma_m4_n8_s4"),
-                     dma8 = variant_index("dma8_m4_n8_s4");
+                     dma8 = variant_index("dma8_m4_n8_s4"), wide = variant_index("dma8k2_m8_n6_s4");
     if (MT == 1) return stream1;
     if (MT == 2) return stream2;
     if (MT <= 4) return which == 0 ? dma4 : stream4;
-    return dma8;
+    if (MT <= 32 || which == 1) return dma8;
+    return wide;
 }
 int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)

@@ -513,14 +513,16 @@ typedef __attribute__((address_space(3))) void lds_void;
 // 16 bytes land at 16 * l of the destination, which is exactly the block layout) through a
 // ring of S stages, so S - 1 k-steps of loads are in flight while the matrix cores work
 // and no VGPRs hold prefetched data.  Needs S * (BM + BN) KiB of dynamic LDS.
-template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1>
 __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__restrict__ a_src,
                                                               const i32x4 *__restrict__ b_src, GemmArgs g) {
     constexpr int NW = WM * WN;  // waves: 4 (one per SIMD) or 8 (two per SIMD)
     static_assert(NW == 4 || NW == 8, "four or eight waves");
     static_assert(S >= 2 && S <= 4, "stages");
+    static_assert(KPB >= 1 && S - KPB >= KPB && S - 2 * KPB <= 2, "k-steps per barrier");
     constexpr int BM = WM * TM, BN = WN * TN, NBLK = BM + BN;
     static_assert(NBLK % NW == 0, "blocks split evenly over the waves");
+    constexpr int AHEAD = S - KPB;  // stages in flight when a group of KPB steps is computed
     static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
     constexpr int PW = NBLK / NW;
     constexpr int BS = b_kstride<PACK>();
@@ -557,25 +559,35 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__res
 #pragma unroll
         for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < S - 1; s++)
+    for (int s = 0; s < AHEAD; s++)
         if (kb + s < ke) issue(kb + s);
-    for (int k = kb; k < ke; k++) {
-        // this wave's loads of step k are done once at most (later steps issued) x PW remain
-        const int later = min(S - 2, ke - 1 - k);
-        if (later >= 2) wait_vmcnt<2 * PW>();
-        else if (later == 1) wait_vmcnt<PW>();
+    // Steps go in groups of KPB per barrier.  Group k0 needs stages k0 .. k0+KPB-1; the
+    // stages issued after them (k0+KPB .. k0+AHEAD-1, those that exist) may stay in flight,
+    // PW loads each.  After the barrier every wave is past the previous group, so its KPB
+    // ring slots take the next stages.
+    for (int k0 = kb; k0 < ke; k0 += KPB) {
+        const int young = max(0, min(AHEAD - KPB, ke - k0 - KPB));
+        if (young >= 2) wait_vmcnt<2 * PW>();
+        else if (young == 1) wait_vmcnt<PW>();
         else wait_vmcnt<0>();
         stage_barrier();
-        if (k + S - 1 < ke) issue(k + S - 1);
-        const i32x4 *cur = ring + (size_t)(((k - kb) % S) * NBLK) * 64;
-        i32x4 b[TN];
 #pragma unroll
-        for (int n = 0; n < TN; n++) b[n] = cur[(BM + wn * TN + n) * 64 + lane];
+        for (int j = 0; j < KPB; j++)
+            if (k0 + AHEAD + j < ke) issue(k0 + AHEAD + j);
 #pragma unroll
-        for (int m = 0; m < TM; m++) {
-            const i32x4 a = cur[(wm * TM + m) * 64 + lane];
+        for (int j = 0; j < KPB; j++) {
+            if (KPB > 1 && k0 + j >= ke) break;
+            const i32x4 *cur = ring + (size_t)(((k0 + j - kb) % S) * NBLK) * 64;
+            i32x4 b[TN];
 #pragma unroll
-            for (int n = 0; n < TN; n++) acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[m][n], 0, 0, 0);
+            for (int n = 0; n < TN; n++) b[n] = cur[(BM + wn * TN + n) * 64 + lane];
+#pragma unroll
+            for (int m = 0; m < TM; m++) {
+                const i32x4 a = cur[(wm * TM + m) * 64 + lane];
+#pragma unroll
+                for (int n = 0; n < TN; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[m][n], 0, 0, 0);
+            }
         }
     }
 #pragma unroll

@@ -66,9 +66,11 @@ struct Db {
     std::deque<int> pending;
     std::vector<uint64_t> tag;
     std::vector<int64_t> t_submit;
+    // one upload stream per database: with the contexts' streams it stays within the
+    // device's hardware queues, so an upload never queues behind a batch's kernels
     hipStream_t upload = nullptr;
-    std::vector<Ctx> ctx;
     std::mutex up_mu;  // serialises uploads on this database's stream
+    std::vector<Ctx> ctx;
 };
 
 struct Server {
@@ -196,8 +198,8 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
         good = ok(hipHostMalloc((void **)&d.h_req, (size_t)d.nslots * d.S, hipHostMallocDefault), s, "pinned requests") &&
                ok(hipMalloc(&d.d_req, (size_t)d.nslots * d.S), s, "request arena") &&
                ok(hipHostMalloc((void **)&d.h_resp, (size_t)d.nslots * d.RL, hipHostMallocMapped), s, "responses") &&
-               ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping") &&
-               ok(hipStreamCreateWithFlags(&d.upload, hipStreamNonBlocking), s, "upload stream");
+               ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping");
+        good = good && ok(hipStreamCreateWithFlags(&d.upload, hipStreamNonBlocking), s, "upload stream");
         d.slot_ev.resize(d.nslots);
         for (auto &ev : d.slot_ev) good = good && ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), s, "event");
         d.tag.resize(d.nslots);
@@ -284,7 +286,8 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
     memcpy(h + ALTO_FRAME_OFFSET, frame, len);
     {
         std::lock_guard<std::mutex> ul(d.up_mu);
-        (void)hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, d.S, hipMemcpyHostToDevice, d.upload);
+        (void)hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, ALTO_FRAME_OFFSET + len, hipMemcpyHostToDevice,
+                             d.upload);
         (void)hipEventRecord(d.slot_ev[slot], d.upload);
     }
     {
