@@ -284,11 +284,28 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
     }
     uint8_t *h = d.h_req + (size_t)slot * d.S;
     memcpy(h + ALTO_FRAME_OFFSET, frame, len);
+    hipError_t ue;
     {
         std::lock_guard<std::mutex> ul(d.up_mu);
-        (void)hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, ALTO_FRAME_OFFSET + len, hipMemcpyHostToDevice,
-                             d.upload);
-        (void)hipEventRecord(d.slot_ev[slot], d.upload);
+        ue = hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, ALTO_FRAME_OFFSET + len, hipMemcpyHostToDevice,
+                            d.upload);
+        if (ue == hipSuccess) ue = hipEventRecord(d.slot_ev[slot], d.upload);
+    }
+    if (ue != hipSuccess) {
+        // the frame never reached the device: report an engine failure for it at once
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = std::string("request upload: ") + hipGetErrorString(ue);
+        alto_completion x{};
+        x.tag = tag;
+        x.db = db;
+        x.status = -1;
+        x.slot = -1;
+        x.t_submit_ns = t0;
+        x.t_done_ns = now_ns();
+        s->completed.push_back(x);
+        d.free_slots.push_back(slot);
+        s->cv_complete.notify_all();
+        return -1;
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);

@@ -204,13 +204,15 @@ def corrupt(frame, rng: np.random.Generator, kp: int) -> bytes:
     return bytes(f)
 
 
-def _stream_device(seed: int, count: int, device):
-    """synthetic.stream(seed, count) as a torch int64 tensor (the u64 bit patterns)."""
+def _stream_device(seeds, count: int, device):
+    """synthetic.stream(seed, count) for each seed of `seeds` (a list), as a (len(seeds), count)
+    torch int64 tensor of the u64 bit patterns."""
     import torch
 
     s64 = lambda v: v - (1 << 64) if v >= (1 << 63) else v  # noqa: E731
-    z = torch.arange(1, count + 1, dtype=torch.int64, device=device)
-    z.mul_(s64(0x9E3779B97F4A7C15)).add_(s64(seed & ((1 << 64) - 1)))
+    sd = torch.tensor([s64(x & ((1 << 64) - 1)) for x in seeds], dtype=torch.int64, device=device)
+    z = torch.arange(1, count + 1, dtype=torch.int64, device=device).unsqueeze(0).repeat(len(seeds), 1)
+    z.mul_(s64(0x9E3779B97F4A7C15)).add_(sd.unsqueeze(1))
     z.bitwise_xor_((z >> 30) & ((1 << 34) - 1))
     z.mul_(s64(0xBF58476D1CE4E5B9))
     z.bitwise_xor_((z >> 27) & ((1 << 37) - 1))
@@ -219,9 +221,11 @@ def _stream_device(seed: int, count: int, device):
     return z
 
 
-def make_requests_device(B: int, rows: int, kp: int, seed: int, stride: int, offset: int = 7, device="cuda"):
+def make_requests_device(B: int, rows: int, kp: int, seed: int, stride: int, offset: int = 7, device="cuda",
+                         group: int = 64):
     """make_requests(B, rows, kp, seed) generated on the GPU straight into a (B, stride) uint8
-    arena, frame c at byte `offset` of row c (the layout of the device entry point)."""
+    arena, frame c at byte `offset` of row c (the layout of the device entry point); `group`
+    requests are generated per pass."""
     import torch
 
     L = request_len(rows, kp)
@@ -229,15 +233,17 @@ def make_requests_device(B: int, rows: int, kp: int, seed: int, stride: int, off
     arena = torch.zeros((B, stride), dtype=torch.uint8, device=device)
     two64_mod_q = (1 << 64) % Q
     hdr = torch.tensor(list(bytes([0]) + kp.to_bytes(4, "little") * 2), dtype=torch.uint8, device=device)
-    for c in range(B):
-        s = seed * 1000003 + c
-        words = torch.empty(2 * kp + rows, dtype=torch.int64, device=device)
-        for i, (sd, cnt, lo) in enumerate(((s * 3 + 1, kp, 0), (s * 3 + 2, kp, kp))):
-            z = _stream_device(sd, cnt, device)
+    arena[:, offset:offset + 9] = hdr
+    for c0 in range(0, B, group):
+        cs = range(c0, min(B, c0 + group))
+        ss = [seed * 1000003 + c for c in cs]
+        words = torch.empty((len(cs), 2 * kp + rows), dtype=torch.int64, device=device)
+        for lo, k in ((0, 1), (kp, 2)):
+            z = _stream_device([s * 3 + k for s in ss], kp, device)
             r = torch.remainder(z, Q)  # z as a signed value; fix up the u64 meaning below
-            r = torch.where(z < 0, torch.remainder(r + two64_mod_q, Q), r)
-            words[lo:lo + cnt] = r
-        words[2 * kp:] = _stream_device(s * 3 + 3, rows, device) & 0xFFFFFFFF
-        arena[c, offset:offset + 9] = hdr
-        arena[c, offset + 9:offset + L] = words.to(torch.int32).view(torch.uint8)
+            words[:, lo:lo + kp] = torch.where(z < 0, torch.remainder(r + two64_mod_q, Q), r)
+            del z, r
+        words[:, 2 * kp:] = _stream_device([s * 3 + 3 for s in ss], rows, device) & 0xFFFFFFFF
+        arena[c0:c0 + len(cs), offset + 9:offset + L] = words.to(torch.int32).view(torch.uint8)
+        del words
     return arena

@@ -201,7 +201,8 @@ typedef struct {
 void *alto_server_new(void *const *engines, int n_engines, int max_batch, int slots_per_db,
                       int contexts_per_db);
 /* Returns 0 (queued), 1 (wrong length) or 9 (no free slot; both completed at once with that
- * status), -1 on bad arguments. */
+ * status), -1 on bad arguments or when the upload cannot be started (then completed at once
+ * with status -1; alto_server_error names the cause). */
 int alto_server_submit(void *srv, int db, const uint8_t *frame, size_t len, uint64_t tag);
 /* Waits up to timeout_ms for completions; returns how many were written to out. */
 int alto_server_poll(void *srv, alto_completion *out, int max, int timeout_ms);
