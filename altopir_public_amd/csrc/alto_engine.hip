@@ -184,6 +184,10 @@ const Variant kVariants[] = {
      prepare_dma<8, 6, 2, 4, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
     {"dma8k2_m8_n6_s4", launch_dma<8, 6, 2, 4, 4, false, 2>, launch_dma<4, 8, 4, 2, 4, true>,
      prepare_dma<8, 6, 2, 4, 4, false, 2>, prepare_dma<4, 8, 4, 2, 4, true>},
+    {"dma8_m4_n12_s4", launch_dma<8, 6, 2, 4, 4, false>, launch_dma<4, 12, 4, 2, 4, true>,
+     prepare_dma<8, 6, 2, 4, 4, false>, prepare_dma<4, 12, 4, 2, 4, true>},
+    {"dma8_m4_n12_s3", launch_dma<8, 6, 2, 4, 3, false>, launch_dma<4, 12, 4, 2, 3, true>,
+     prepare_dma<8, 6, 2, 4, 3, false>, prepare_dma<4, 12, 4, 2, 3, true>},
     {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
      prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,
@@ -309,6 +313,7 @@ struct Context {
         uint64_t used;
     };
     std::vector<GraphEntry> graphs;
+    int graph_mode = -1;  // -1: the engine's setting, 0: eager, 1: capture and replay
     uint64_t tick = 0;
     static constexpr size_t kMaxGraphs = 64;
 
@@ -491,7 +496,7 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
 // once per distinct batch and replayed.
 int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
                            hipStream_t st) {
-    if (!e->use_graphs) return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
+    if (graph_mode == 0 || (graph_mode < 0 && !e->use_graphs)) return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
     const int MT = (int)(round_up(B, 4) / 4);
     const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT);
     timed_valid = false;
@@ -843,6 +848,14 @@ void alto_context_free(void *c) {
 }
 
 void *alto_engine_context(void *p) { return p ? ((Engine *)p)->ctx0 : nullptr; }
+
+int alto_context_set_graphs(void *c, int mode) {
+    auto *ctx = (Context *)c;
+    if (!ctx || mode < -1 || mode > 1) return -1;
+    if (mode == 0) ctx->drop_graphs();
+    ctx->graph_mode = mode;
+    return 0;
+}
 
 int alto_context_reserve(void *c, size_t B) {
     auto *ctx = (Context *)c;

@@ -383,4 +383,87 @@ void alto_server_free(void *p) {
     delete s;
 }
 
+int alto_server_set_graphs(void *p, int mode) {
+    auto *s = (Server *)p;
+    if (!s || mode < -1 || mode > 1) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto &dp : s->dbs)
+        for (auto &c : dp->ctx) {
+            if (c.busy) (void)hipStreamSynchronize(c.stream);
+            if (alto_context_set_graphs(c.ctx, mode) != 0) return -1;
+        }
+    return 0;
+}
+
+int alto_server_replay(void *p, int db, const uint8_t *const *frames, const size_t *lens, int nframes,
+                       long long total, int threads, int window, alto_replay_result *out) {
+    auto *s = (Server *)p;
+    if (!s || db < 0 || db >= (int)s->dbs.size() || !frames || !lens || nframes <= 0 || total <= 0 ||
+        threads <= 0 || window <= 0 || !out)
+        return -1;
+    alto_server_stats st0{}, st1{};
+    (void)alto_server_get_stats(s, &st0);
+    std::mutex wmu;
+    std::condition_variable wcv;
+    long long outstanding = 0;
+    std::vector<double> lat_ms((size_t)total, -1.0);
+    long long got = 0, answered = 0, rejected = 0;
+    const int64_t t0 = now_ns();
+    std::thread collector([&] {
+        std::vector<alto_completion> buf(512);
+        while (got < total) {
+            const int k = alto_server_poll(s, buf.data(), (int)buf.size(), 20);
+            for (int i = 0; i < k; i++) {
+                const alto_completion &c = buf[i];
+                if (c.status == 0) {
+                    answered++;
+                    (void)alto_server_release(s, c.db, c.slot);
+                } else {
+                    rejected++;
+                }
+                if (c.tag < (uint64_t)total) lat_ms[c.tag] = (c.t_done_ns - c.t_submit_ns) / 1e6;
+                got++;
+            }
+            if (k > 0) {
+                std::lock_guard<std::mutex> lk(wmu);
+                outstanding -= k;
+                wcv.notify_all();
+            }
+        }
+    });
+    std::vector<std::thread> subs;
+    for (int t = 0; t < threads; t++)
+        subs.emplace_back([&, t] {
+            for (long long i = t; i < total; i += threads) {
+                {
+                    std::unique_lock<std::mutex> lk(wmu);
+                    wcv.wait(lk, [&] { return outstanding < window; });
+                    outstanding++;
+                }
+                const int f = (int)(i % nframes);
+                (void)alto_server_submit(s, db, frames[f], lens[f], (uint64_t)i);
+            }
+        });
+    for (auto &t : subs) t.join();
+    collector.join();
+    const double wall = (now_ns() - t0) / 1e9;
+    (void)alto_server_get_stats(s, &st1);
+    std::vector<double> v;
+    for (double x : lat_ms)
+        if (x >= 0) v.push_back(x);
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(q / 100.0 * v.size()))]; };
+    out->answered = answered;
+    out->rejected = rejected;
+    out->batches = st1.batches - st0.batches;
+    out->wall_s = wall;
+    out->qps = wall > 0 ? answered / wall : 0.0;
+    out->mean_batch = out->batches ? (double)(st1.answered_in_batches - st0.answered_in_batches) / out->batches : 0.0;
+    out->p50_ms = pct(50);
+    out->p90_ms = pct(90);
+    out->p99_ms = pct(99);
+    out->max_ms = v.empty() ? 0.0 : v.back();
+    return 0;
+}
+
 }  // extern "C"

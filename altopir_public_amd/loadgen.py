@@ -5,8 +5,9 @@ answered rate and the latency percentiles (scheduled send -> completion) are rep
 
     python -m altopir_public_amd.loadgen --gib 1 16 32 --rates 50 200 500 1000 2000 --duration 60
 
-Each database holds R = gib * 2^30 / 32768 records of 32,768 uniform bytes (seeds 0, 1, 2, ...),
-generated in device memory, with its own packing matrix, tail keys and prefixes.  Requests are
+Each database holds R = gib * 2^30 / C records of C uniform bytes (C = --record-bytes, default
+32,768; seeds 0, 1, 2, ...), generated in device memory, with its own packing matrix, tail keys and
+prefixes.  Requests are
 drawn from a pool of distinct valid frames per database (seed 5) and 1% are corrupted with
 workload.corrupt.  One JSON line per rate is printed on stdout.
 """
@@ -70,6 +71,7 @@ def main(argv=None) -> int:
     ap.add_argument("--slots", type=int, default=256, help="request slots per database")
     ap.add_argument("--contexts", type=int, default=2, help="batches in flight per database")
     ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--record-bytes", type=int, default=32768, help="record size (a multiple of n = 2048)")
     args = ap.parse_args(argv)
 
     import torch
@@ -78,7 +80,7 @@ def main(argv=None) -> int:
 
     assert torch.cuda.is_available(), "the load generator needs a HIP device"
     torch.cuda.set_device(0)
-    cols, n, tg = 32768, 2048, 4
+    cols, n, tg = args.record_bytes, 2048, 4
     engines, inputs = build_engines(args.gib, cols, n, tg, args.threads)
     pools = [frame_pool(inp, args.pool, seed=args.seed * 1000 + i) for i, inp in enumerate(inputs)]
     srv = serve.Server(engines, max_batch=args.max_batch, slots=args.slots, contexts=args.contexts)

@@ -28,6 +28,12 @@ class Stats(C.Structure):
                 ("answered_in_batches", C.c_longlong), ("max_batch_seen", C.c_longlong)]
 
 
+class ReplayResult(C.Structure):
+    _fields_ = [("answered", C.c_longlong), ("rejected", C.c_longlong), ("batches", C.c_longlong),
+                ("wall_s", C.c_double), ("qps", C.c_double), ("mean_batch", C.c_double), ("p50_ms", C.c_double),
+                ("p90_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double)]
+
+
 def _bind(lib):
     if getattr(lib, "_serve_bound", False):
         return lib
@@ -48,6 +54,11 @@ def _bind(lib):
     lib.alto_server_error.argtypes = [P]
     lib.alto_server_free.restype = None
     lib.alto_server_free.argtypes = [P]
+    lib.alto_server_set_graphs.restype = C.c_int
+    lib.alto_server_set_graphs.argtypes = [P, C.c_int]
+    lib.alto_server_replay.restype = C.c_int
+    lib.alto_server_replay.argtypes = [P, C.c_int, P, P, C.c_int, C.c_longlong, C.c_int, C.c_int,
+                                       C.POINTER(ReplayResult)]
     lib._serve_bound = True
     return lib
 
@@ -75,6 +86,11 @@ class Server:
             raise RuntimeError("alto_server_new failed (see stderr)")
         self._buf = (Completion * 512)()
 
+    def set_graphs(self, mode: int) -> None:
+        """1: capture and replay each distinct batch, 0: eager launches, -1: follow the engines."""
+        if self.lib.alto_server_set_graphs(self.ptr, mode) != 0:
+            raise RuntimeError("alto_server_set_graphs failed")
+
     def contexts(self, db: int = 0) -> int:
         return self.lib.alto_server_contexts(self.ptr, db)
 
@@ -95,6 +111,25 @@ class Server:
                     resp = C.string_at(c.response, c.response_len)
                 self.lib.alto_server_release(self.ptr, c.db, c.slot)
             out.append(Done(c.tag, c.db, c.status, resp, c.t_submit_ns, c.t_done_ns, c.batch))
+        return out
+
+    def replay(self, frames, db: int = 0, total: int = 1024, threads: int = 8, window: int = 384) -> dict:
+        """Native load driver (alto_server_replay): `threads` submitter threads keep up to
+        `window` requests of database `db` outstanding (frames round-robin) until `total` are
+        done; returns the answered rate and latency percentiles."""
+        arrs = [np.frombuffer(f, np.uint8) if isinstance(f, (bytes, bytearray)) else np.ascontiguousarray(f)
+                for f in frames]
+        fp = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        ln = (C.c_size_t * len(arrs))(*[len(a) for a in arrs])
+        r = ReplayResult()
+        rc = self.lib.alto_server_replay(self.ptr, db, C.addressof(fp), C.addressof(ln), len(arrs), total, threads,
+                                         window, C.byref(r))
+        if rc != 0:
+            raise RuntimeError("alto_server_replay: bad arguments")
+        out = {f: getattr(r, f) for f, _ in ReplayResult._fields_}
+        for k in ("wall_s", "qps", "mean_batch", "p50_ms", "p90_ms", "p99_ms", "max_ms"):
+            out[k] = round(out[k], 4)
+        out.update(requests=total, threads=threads, window=window)
         return out
 
     def stats(self) -> dict:

@@ -111,8 +111,13 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2, help="request seed")
     ap.add_argument("--tune", action="store_true",
                     help="time every tile-shape variant per batch size first and keep the fastest")
-    ap.add_argument("--serve-batches", type=int, default=8,
+    ap.add_argument("--serve-batches", type=int, default=16,
                     help="batches' worth of requests pushed through the serving loop (0: skip)")
+    ap.add_argument("--serve-threads", type=int, default=16, help="submitter threads of the served form")
+    ap.add_argument("--serve-contexts", type=int, default=2, help="batches in flight in the served form")
+    ap.add_argument("--serve-window", type=int, default=6, help="outstanding requests, in batches")
+    ap.add_argument("--serve-graphs", type=int, default=-1, choices=[-1, 0, 1],
+                    help="graph mode of the served form's contexts (-1 follow the engine)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -269,10 +274,13 @@ def main() -> int:
         from altopir_public_amd import serve
 
         frames = [np.ascontiguousarray(f[7:7 + L]) for f in slots.cpu().numpy()]
-        srv = serve.Server([eng], max_batch=B, slots=3 * B, contexts=2)
-        serve.saturate(srv, frames, total=2 * B, threads=8, window=3 * B)  # warm-up
-        served = serve.saturate(srv, frames, total=args.serve_batches * B, threads=8, window=3 * B)
-        served["form"] = "alto_server_* loop, host frames copied into pinned slots by 8 submitter threads"
+        win = args.serve_window * B
+        srv = serve.Server([eng], max_batch=B, slots=win, contexts=args.serve_contexts)
+        srv.set_graphs(args.serve_graphs)
+        srv.replay(frames, total=2 * B, threads=args.serve_threads, window=win)  # warm-up
+        served = srv.replay(frames, total=args.serve_batches * B, threads=args.serve_threads, window=win)
+        served["form"] = (f"alto_server_* loop driven by alto_server_replay: {args.serve_threads} native submitter "
+                          f"threads copy host frames into pinned slots, up to {win} requests outstanding")
         served["contexts"] = srv.contexts(0)
         srv.close()
         log(f"[bench] served form: {served}")

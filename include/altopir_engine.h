@@ -119,6 +119,9 @@ void *alto_context_new(void *e);
 void alto_context_free(void *ctx);
 void *alto_engine_context(void *e);
 int alto_context_reserve(void *ctx, size_t B);
+/* Graph mode of one context: 1 capture and replay, 0 eager (drops its graphs), -1 follow
+ * the engine's alto_set_graphs (the default).  Returns 0, -1 on bad arguments. */
+int alto_context_set_graphs(void *ctx, int mode);
 
 /* alto_answer_device on a context, with frames and outputs gathered through optional
  * device index arrays: frame c at slots + slot_index[c] * slot_stride + ALTO_FRAME_OFFSET,
@@ -211,6 +214,22 @@ int alto_server_contexts(void *srv, int db);
 int alto_server_get_stats(void *srv, alto_server_stats *out);
 const char *alto_server_error(void *srv);
 void alto_server_free(void *srv);
+
+/* Native load driver for the serving loop, standing in for a server's connection readers:
+ * `threads` submitter threads send `total` requests for database `db` (frame i % nframes,
+ * tag i), keeping at most `window` outstanding, while a collector polls completions and
+ * releases their slots.  The caller must not poll the server meanwhile.  Fills `out` with
+ * the answered rate over the run and the submit-to-completion latency percentiles.
+ * Returns 0, -1 on bad arguments. */
+typedef struct {
+    long long answered, rejected, batches;
+    double wall_s, qps, mean_batch, p50_ms, p90_ms, p99_ms, max_ms;
+} alto_replay_result;
+int alto_server_replay(void *srv, int db, const uint8_t *const *frames, const size_t *lens, int nframes,
+                       long long total, int threads, int window, alto_replay_result *out);
+/* Graph mode of every context of the server (see alto_context_set_graphs; the server's
+ * contexts start in -1, following their engine).  Returns 0, -1 on bad arguments. */
+int alto_server_set_graphs(void *srv, int mode);
 
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */

@@ -295,3 +295,36 @@ def test_packed_product_matches_the_oracle(hip_device, rows, cols):
         orc.orc_matmul_vec_packed(want.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols, bc)
         getattr(lib, fn)(got.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols)
         assert np.array_equal(got, want), fn
+
+
+def test_tuner_records_a_variant_per_bucket_and_keeps_the_bytes(hip_device, small_inputs, tmp_path):
+    # the tuner times every variant on device-resident batches and records the fastest per
+    # batch-size bucket; answers after tuning equal the oracle's, and a saved table applies
+    # to a fresh engine
+    from altopir_public_amd import autotune
+
+    inp = small_inputs
+    eng = Engine.from_inputs(inp)
+    eng.set_graphs(False)
+    table = autotune.tune(eng, inp, [1, 6, 40], seed=3, reps=1)
+    names = autotune.variant_names()
+    assert set(table) == {autotune.bucket(b) for b in (1, 6, 40)}
+    for ent in table.values():
+        assert ent["scan"][0] in names and ent["pack"][0] in names
+        assert eng._lib.alto_tune_get(eng.ptr, autotune.SCAN, ent["B"]) == names.index(ent["scan"][0])
+    reqs = workload.make_requests(40, inp.rows, inp.kp, seed=31)
+    out_g = np.zeros((40, inp.response_len), np.uint8)
+    st_g = np.full(40, -1, np.int32)
+    answered, stage = eng.answer(reqs, out_g, st_g)
+    out_o = np.zeros_like(out_g)
+    st_o = np.full(40, -1, np.int32)
+    r_o = oc.oracle_answer(inp, reqs, out_o, st_o)
+    assert_same((answered, st_g, out_g, stage), (r_o, st_o, out_o), "after tuning")
+    path = tmp_path / "tune.json"
+    autotune.save(table, path, inp.rows, inp.cols)
+    fresh = Engine.from_inputs(inp)
+    assert autotune.load(fresh, path, inp.rows, inp.cols) == len(table)
+    for ent in table.values():
+        assert fresh._lib.alto_tune_get(fresh.ptr, autotune.PACK, ent["B"]) == names.index(ent["pack"][0])
+    fresh.close()
+    eng.close()

@@ -112,3 +112,40 @@ def test_poisson_load_reports_latency_and_rejects_corrupt_frames(hip_device, two
     assert s["p99_ms"] >= s["p50_ms"] > 0
     srv.close()
     eng.close()
+
+
+def test_saturated_loop_answers_everything_in_full_batches(hip_device, two_dbs):
+    # offered load above capacity from several submitter threads: every request is answered
+    # and the loop forms batches larger than one
+    from altopir_public_amd import serve
+
+    inp = two_dbs[0]
+    eng = Engine.from_inputs(inp)
+    srv = Server([eng], max_batch=32, slots=96, contexts=2)
+    frames = [np.ascontiguousarray(f) for f in workload.make_requests(16, inp.rows, inp.kp, seed=77)]
+    res = serve.saturate(srv, frames, total=320, threads=4, window=96)
+    assert res["answered"] == 320 and res["statuses"] == {0: 320}
+    assert res["mean_batch"] > 1 and res["p99_ms"] >= res["p50_ms"] > 0
+    nat = srv.replay(frames, total=640, threads=8, window=96)  # the native driver
+    assert nat["answered"] == 640 and nat["rejected"] == 0 and nat["batches"] >= 20
+    assert nat["mean_batch"] > 1 and nat["max_ms"] >= nat["p99_ms"] >= nat["p50_ms"] > 0
+    srv.close()
+    eng.close()
+
+
+def test_load_generator_end_to_end(hip_device, capsys):
+    # two small resident databases, Poisson arrivals with 5% corrupted frames: each corrupted
+    # frame is rejected with the decoder's status, every valid one answered
+    import json
+
+    from altopir_public_amd import loadgen
+
+    assert loadgen.main(["--gib", "0.0078125", "0.015625", "--record-bytes", "4096", "--rates", "300",
+                         "--duration", "1", "--corrupt", "0.05", "--pool", "4", "--max-batch", "16",
+                         "--slots", "64"]) == 0
+    lines = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")]
+    assert len(lines) == 1
+    r = lines[0]
+    assert r["rejected_as_expected"] and r["sent"] == 300
+    assert r["answered"] + sum(r["rejected"].values()) == 300
+    assert r["p99_ms"] >= r["p50_ms"] > 0
