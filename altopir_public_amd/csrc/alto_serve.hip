@@ -41,6 +41,11 @@ int64_t now_ns() {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Frames up to this size are read by the kernels straight from the pinned slots by default:
+// for small frames the per-frame copy calls cost more than the PCIe reads; large frames are
+// copied at submit so that their transfer overlaps earlier batches' work.
+constexpr size_t kZeroCopyMaxFrame = 512 << 10;
+
 struct Server;
 
 struct Ctx {
@@ -59,7 +64,11 @@ struct Db {
     void *engine = nullptr;
     size_t L = 0, RL = 0, S = 0;
     int nslots = 0;
-    uint8_t *h_req = nullptr, *d_req = nullptr;      // nslots x S
+    uint8_t *h_req = nullptr, *d_req = nullptr;      // nslots x S (pinned, mapped / device arena)
+    uint8_t *h_req_dev = nullptr;                    // device view of h_req
+    // true: the batch's kernels read the frames from the pinned slots over PCIe; false: each
+    // frame is copied into d_req at submit
+    std::atomic<bool> zero_copy{false};
     uint8_t *h_resp = nullptr, *d_resp = nullptr;    // nslots x RL, mapped pinned
     std::vector<hipEvent_t> slot_ev;
     std::vector<int> free_slots;
@@ -119,14 +128,15 @@ void dispatcher(Server *s, int di) {
         c.busy = true;
         const int B = (int)c.batch.size();
         lk.unlock();
+        const bool zc = d.zero_copy;
         for (int i = 0; i < B; i++) {
             c.h_idx[i] = c.batch[i];
             c.h_status[i] = 0;
-            (void)hipStreamWaitEvent(c.stream, d.slot_ev[c.batch[i]], 0);
+            if (!zc) (void)hipStreamWaitEvent(c.stream, d.slot_ev[c.batch[i]], 0);
         }
         bool good = ok(hipMemcpyAsync(c.d_idx, c.h_idx, B * sizeof(int), hipMemcpyHostToDevice, c.stream), s, "index");
-        const int rc = good ? alto_answer_gather(c.ctx, d.d_req, d.S, c.d_idx, B, d.d_resp, d.RL, c.d_idx, c.d_status,
-                                                 c.stream)
+        const int rc = good ? alto_answer_gather(c.ctx, zc ? d.h_req_dev : d.d_req, d.S, c.d_idx, B, d.d_resp, d.RL,
+                                                 c.d_idx, c.d_status, c.stream)
                             : -1;
         if (rc != 0) s->err = std::string("alto_answer_gather: ") + alto_last_error();
         (void)hipEventRecord(c.done, c.stream);
@@ -195,7 +205,9 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
         d.RL = alto_response_len(d.engine);
         d.S = round_up(ALTO_FRAME_OFFSET + d.L, 256);
         d.nslots = slots_per_db;
-        good = ok(hipHostMalloc((void **)&d.h_req, (size_t)d.nslots * d.S, hipHostMallocDefault), s, "pinned requests") &&
+        d.zero_copy = d.L <= kZeroCopyMaxFrame;
+        good = ok(hipHostMalloc((void **)&d.h_req, (size_t)d.nslots * d.S, hipHostMallocMapped), s, "pinned requests") &&
+               ok(hipHostGetDevicePointer((void **)&d.h_req_dev, d.h_req, 0), s, "request mapping") &&
                ok(hipMalloc(&d.d_req, (size_t)d.nslots * d.S), s, "request arena") &&
                ok(hipHostMalloc((void **)&d.h_resp, (size_t)d.nslots * d.RL, hipHostMallocMapped), s, "responses") &&
                ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping");
@@ -284,8 +296,8 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
     }
     uint8_t *h = d.h_req + (size_t)slot * d.S;
     memcpy(h + ALTO_FRAME_OFFSET, frame, len);
-    hipError_t ue;
-    {
+    hipError_t ue = hipSuccess;
+    if (!d.zero_copy) {
         std::lock_guard<std::mutex> ul(d.up_mu);
         ue = hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, ALTO_FRAME_OFFSET + len, hipMemcpyHostToDevice,
                             d.upload);
@@ -381,6 +393,20 @@ void alto_server_free(void *p) {
         if (d.h_resp) (void)hipHostFree(d.h_resp);
     }
     delete s;
+}
+
+int alto_server_set_upload(void *p, int mode) {
+    auto *s = (Server *)p;
+    if (!s || mode < -1 || mode > 1) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto &dp : s->dbs) {
+        if (!dp->pending.empty()) return -1;
+        for (auto &c : dp->ctx)
+            if (c.busy) return -1;
+        if (dp->upload) (void)hipStreamSynchronize(dp->upload);
+    }
+    for (auto &dp : s->dbs) dp->zero_copy = mode < 0 ? dp->L <= kZeroCopyMaxFrame : mode == 1;
+    return 0;
 }
 
 int alto_server_set_graphs(void *p, int mode) {

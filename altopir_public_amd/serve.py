@@ -54,6 +54,8 @@ def _bind(lib):
     lib.alto_server_error.argtypes = [P]
     lib.alto_server_free.restype = None
     lib.alto_server_free.argtypes = [P]
+    lib.alto_server_set_upload.restype = C.c_int
+    lib.alto_server_set_upload.argtypes = [P, C.c_int]
     lib.alto_server_set_graphs.restype = C.c_int
     lib.alto_server_set_graphs.argtypes = [P, C.c_int]
     lib.alto_server_replay.restype = C.c_int
@@ -90,6 +92,12 @@ class Server:
         """1: capture and replay each distinct batch, 0: eager launches, -1: follow the engines."""
         if self.lib.alto_server_set_graphs(self.ptr, mode) != 0:
             raise RuntimeError("alto_server_set_graphs failed")
+
+    def set_upload(self, mode: int) -> None:
+        """0: copy each frame into HBM at submit; 1: kernels read the pinned slots over PCIe;
+        -1: 1 for frames up to 512 KiB, else 0 (the default)."""
+        if self.lib.alto_server_set_upload(self.ptr, mode) != 0:
+            raise RuntimeError("alto_server_set_upload failed (requests pending or bad mode)")
 
     def contexts(self, db: int = 0) -> int:
         return self.lib.alto_server_contexts(self.ptr, db)

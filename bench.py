@@ -118,6 +118,9 @@ def main() -> int:
     ap.add_argument("--serve-window", type=int, default=6, help="outstanding requests, in batches")
     ap.add_argument("--serve-graphs", type=int, default=-1, choices=[-1, 0, 1],
                     help="graph mode of the served form's contexts (-1 follow the engine)")
+    ap.add_argument("--serve-upload", type=int, default=-1, choices=[-1, 0, 1],
+                    help="served form: 0 copy frames into HBM at submit, 1 kernels read the pinned slots, "
+                         "-1 by frame size")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -277,6 +280,7 @@ def main() -> int:
         win = args.serve_window * B
         srv = serve.Server([eng], max_batch=B, slots=win, contexts=args.serve_contexts)
         srv.set_graphs(args.serve_graphs)
+        srv.set_upload(args.serve_upload)
         srv.replay(frames, total=2 * B, threads=args.serve_threads, window=win)  # warm-up
         served = srv.replay(frames, total=args.serve_batches * B, threads=args.serve_threads, window=win)
         served["form"] = (f"alto_server_* loop driven by alto_server_replay: {args.serve_threads} native submitter "
