@@ -319,13 +319,17 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
         s->cv_complete.notify_all();
         return -1;
     }
+    bool wake = false;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         d.tag[slot] = tag;
         d.t_submit[slot] = t0;
         d.pending.push_back(slot);
+        // the dispatcher waits for a free context; when all are busy, the completion of one
+        // wakes it and it then takes everything pending
+        for (const Ctx &c : d.ctx) wake = wake || !c.busy;
     }
-    s->cv_dispatch.notify_all();
+    if (wake) s->cv_dispatch.notify_all();
     return 0;
 }
 
@@ -347,8 +351,7 @@ int alto_server_release(void *p, int db, int slot) {
     auto *s = (Server *)p;
     if (!s || db < 0 || db >= (int)s->dbs.size() || slot < 0 || slot >= s->dbs[db]->nslots) return -1;
     std::lock_guard<std::mutex> lk(s->mu);
-    s->dbs[db]->free_slots.push_back(slot);
-    s->cv_dispatch.notify_all();
+    s->dbs[db]->free_slots.push_back(slot);  // no dispatcher waits on slots
     return 0;
 }
 
