@@ -64,6 +64,7 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict
     lib = _lib.load()
     stage = 2 if which == SCAN else 4  # index in alto_last_timings
     out = {}
+    fusion = lib.alto_set_fusion(engine.ptr, 0)  # each contraction timed in a launch of its own
     saved = lib.alto_tune_get(engine.ptr, which, batch.B)
     mt = (batch.B + 3) // 4
     for idx, name in enumerate(variant_names()):
@@ -76,6 +77,7 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict
             ts.append(engine.last_timings()[stage])
         out[name] = float(np.median(ts[1:]))
     lib.alto_tune_set(engine.ptr, which, batch.B, saved)
+    lib.alto_set_fusion(engine.ptr, fusion)
     return out
 
 

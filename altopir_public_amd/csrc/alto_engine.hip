@@ -58,13 +58,27 @@ int choose_splits(int KC, long long tiles, int target, int min_steps) {
     return gz;
 }
 
-template <int MB, int NBT, int U, bool PACK>
-void launch_stream(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+// Workgroup grid of one contraction (sets g.gx, g.gy, g.gz; returns the workgroup count):
+// streaming shapes split the record range until about 2048 workgroups, ring shapes until
+// 256 (one per CU); the packing contraction is never split (KP fits one i32 range).
+template <int MB, int NBT>
+int plan_stream(GemmArgs &g, bool pack) {
     g.gx = ceil_div(g.NT, 4 * NBT);
     g.gy = ceil_div(g.MT, MB);
-    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 2048, 16);
-    else g.gz = 1;
-    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    g.gz = pack ? 1 : choose_splits(g.KC, (long long)g.gx * g.gy, 2048, 16);
+    return 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+}
+template <int TM, int TN, int WM, int WN>
+int plan_ring(GemmArgs &g, bool pack) {
+    g.gx = ceil_div(g.NT, WN * TN);
+    g.gy = ceil_div(g.MT, WM * TM);
+    g.gz = pack ? 1 : choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
+    return 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+}
+
+template <int MB, int NBT, int U, bool PACK>
+void launch_stream(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    const int grid = plan_stream<MB, NBT>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((stream_kernel<MB, NBT, U, PACK, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
                            (const i32x4 *)b, g);
@@ -104,11 +118,7 @@ template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1>
 void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     constexpr int NBLK = WM * TM + WN * TN;
     constexpr size_t lds = (size_t)S * NBLK * 1024;
-    g.gx = ceil_div(g.NT, WN * TN);
-    g.gy = ceil_div(g.MT, WM * TM);
-    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
-    else g.gz = 1;
-    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB>), dim3(grid), dim3(64 * WM * WN), lds,
                            st, (const i32x4 *)a, (const i32x4 *)b, g);
@@ -199,6 +209,63 @@ const Variant kVariants[] = {
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
+// Scan and packing in one launch (scan_pack_kernel) for the shape pairs of the built-in
+// policy; other pairs run as two launches.
+typedef void (*FusedFn)(GemmArgs, const int8_t *, const int8_t *, GemmArgs, const int8_t *, const int8_t *,
+                        hipStream_t);
+
+template <int MB, int NBT, int U, int MBp, int NBTp, int Up>
+void launch_fused_stream(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya,
+                         const int8_t *u, hipStream_t st) {
+    const int sb = plan_stream<MB, NBT>(gs, false), pb = plan_stream<MBp, NBTp>(gp, true);
+    using P = StreamShape<MBp, NBTp, Up, true, false>;
+    if (gs.gz > 1)
+        hipLaunchKernelGGL((scan_pack_kernel<StreamShape<MB, NBT, U, false, true>, P>), dim3(sb + pb), dim3(256), 0, st,
+                           (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+    else
+        hipLaunchKernelGGL((scan_pack_kernel<StreamShape<MB, NBT, U, false, false>, P>), dim3(sb + pb), dim3(256), 0,
+                           st, (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+}
+
+template <int TM, int TN, int WM, int WN, int S, int KPB, int TMp, int TNp, int WMp, int WNp, int Sp, int KPBp>
+struct FusedRing {
+    using SA = DmaShape<TM, TN, WM, WN, S, false, true, KPB>;
+    using SN = DmaShape<TM, TN, WM, WN, S, false, false, KPB>;
+    using P = DmaShape<TMp, TNp, WMp, WNp, Sp, true, false, KPBp>;
+    static constexpr size_t kLds = SA::kLds > P::kLds ? SA::kLds : P::kLds;
+    static void prepare() {
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SA, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SN, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+    }
+    static void launch(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya,
+                       const int8_t *u, hipStream_t st) {
+        const int sb = plan_ring<TM, TN, WM, WN>(gs, false), pb = plan_ring<TMp, TNp, WMp, WNp>(gp, true);
+        if (gs.gz > 1)
+            hipLaunchKernelGGL((scan_pack_kernel<SA, P>), dim3(sb + pb), dim3(SA::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+        else
+            hipLaunchKernelGGL((scan_pack_kernel<SN, P>), dim3(sb + pb), dim3(SN::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+    }
+};
+using FusedRing8 = FusedRing<4, 8, 4, 2, 4, 1, 4, 8, 4, 2, 4, 1>;
+using FusedRingWide = FusedRing<8, 6, 2, 4, 4, 2, 4, 8, 4, 2, 4, 1>;
+
+struct FusedPair {
+    const char *scan, *pack;
+    FusedFn launch;
+    PrepareFn prepare;
+};
+const FusedPair kFused[] = {
+    {"stream_m1_n4_u4", "stream_m1_n4_u4", launch_fused_stream<1, 4, 4, 1, 4, 4>, prepare_none},
+    {"stream_m2_n4_u4", "stream_m2_n4_u4", launch_fused_stream<2, 4, 4, 2, 4, 4>, prepare_none},
+    {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
+    {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
+};
+constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
+
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
 // from sweeps over batch sizes 1-2048 on 1 and 8 GiB databases: streaming kernels up to 8
 // queries, the 4-wave ring for 9-16 queries (scan), the 8-wave 16 x 16-tile ring up to 128
@@ -219,6 +286,13 @@ int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)
         if (!strcmp(kVariants[i].name, name)) return i;
     return 0;
+}
+
+// Index into kFused of the pair (scan variant vs, packing variant vp), or -1.
+int fused_index(int vs, int vp) {
+    for (int i = 0; i < kNumFused; i++)
+        if (variant_index(kFused[i].scan) == vs && variant_index(kFused[i].pack) == vp) return i;
+    return -1;
 }
 
 // Batch-size buckets of the tuning table: A tiles (4 queries each) 1, 2, 3-4, 5-8, ...,
@@ -252,6 +326,7 @@ struct Engine {
     uint32_t k128e = 0, ycorr0 = 0;
     int tuned[2][kBuckets];      // [scan, pack][bucket] -> variant, -1 = built-in policy
     bool use_graphs = true;
+    bool fuse = false;           // one launch for scan and packing where the shape pair allows
     Context *ctx0 = nullptr;     // the context of alto_answer / alto_answer_device
 
     Engine() {
@@ -263,6 +338,8 @@ struct Engine {
         const int v = tuned[which][bucket_of(MT)];
         return v >= 0 ? v : default_variant(which, MT);
     }
+    // kFused index used for a batch of MT A tiles, or -1 for two launches
+    int fused_for(int MT) const { return fuse ? fused_index(variant_for(0, MT), variant_for(1, MT)) : -1; }
     size_t request_len() const { return 9 + 4 * (2 * kp + rows); }
     size_t response_len() const { return outputs * part_len; }
 };
@@ -308,7 +385,7 @@ struct Context {
         const int *oidx;
         int *status;
         hipStream_t st;
-        int vs, vp;
+        int vs, vp, fz;
         hipGraphExec_t exec;
         uint64_t used;
     };
@@ -425,37 +502,50 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     if (timed && record_start) (void)hipEventRecord(next_set()[0], st);
     hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, st_dev, (uint32_t)E.kp, qsum, ysum);
     if (timed) (void)hipEventRecord(ev[1], st);
-    // scan: query limbs, then the contraction into acc
-    hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((E.KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
-                       st, fs, (size_t)(2 * E.kp), (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
-    if (timed) (void)hipEventRecord(ev[2], st);
-    {
-        GemmArgs g{};
-        g.KC = (int)E.KC;
-        g.MT = (int)MT;
-        g.NT = (int)E.CT;
-        g.C = (int)E.cols;
-        g.out = acc;
-        kVariants[E.variant_for(0, (int)MT)].scan(g, qa, E.db, st);
-    }
-    if (timed) (void)hipEventRecord(ev[3], st);
-    // packing: key limbs, then the contraction into beta
-    hipLaunchKernelGGL((limb_prep_kernel<false>), dim3((unsigned)((E.KCp * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
-                       st, fs, (size_t)0, (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
-    if (timed) (void)hipEventRecord(ev[4], st);
-    {
-        GemmArgs g{};
-        g.KC = (int)E.KCp;
-        g.MT = (int)MT;
-        g.NT = (int)(4 * E.CT);
-        g.C = (int)E.cols;
-        g.out = beta;
-        g.ucorr = E.ucorr;
-        g.ysum = ysum;
-        g.k128e = E.k128e;
-        g.ycorr0 = E.ycorr0;
-        kVariants[E.variant_for(1, (int)MT)].pack(g, ya, E.u, st);
+    GemmArgs gs{};  // scan: query limbs x database -> acc
+    gs.KC = (int)E.KC;
+    gs.MT = (int)MT;
+    gs.NT = (int)E.CT;
+    gs.C = (int)E.cols;
+    gs.out = acc;
+    GemmArgs gp{};  // packing: key limbs x packing matrix -> beta
+    gp.KC = (int)E.KCp;
+    gp.MT = (int)MT;
+    gp.NT = (int)(4 * E.CT);
+    gp.C = (int)E.cols;
+    gp.out = beta;
+    gp.ucorr = E.ucorr;
+    gp.ysum = ysum;
+    gp.k128e = E.k128e;
+    gp.ycorr0 = E.ycorr0;
+    const dim3 scan_prep_grid((unsigned)((E.KC * 4 + 255) / 256), (unsigned)Bp),
+        pack_prep_grid((unsigned)((E.KCp * 4 + 255) / 256), (unsigned)Bp);
+    const int fz = E.fused_for((int)MT);
+    if (fz >= 0) {
+        // both limb preparations, then one launch for both contractions; the stage times
+        // show the pair under "scan" (packing preparation and packing read as zero)
+        hipLaunchKernelGGL((limb_prep_kernel<true>), scan_prep_grid, dim3(256), 0, st, fs, (size_t)(2 * E.kp),
+                           (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
+        hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
+                           (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
+        if (timed) (void)hipEventRecord(ev[2], st);
+        kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
+        if (timed) {
+            (void)hipEventRecord(ev[3], st);
+            (void)hipEventRecord(ev[4], st);
+        }
+    } else {
+        hipLaunchKernelGGL((limb_prep_kernel<true>), scan_prep_grid, dim3(256), 0, st, fs, (size_t)(2 * E.kp),
+                           (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
+        if (timed) (void)hipEventRecord(ev[2], st);
+        kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
+        if (timed) (void)hipEventRecord(ev[3], st);
+        hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
+                           (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
+        if (timed) (void)hipEventRecord(ev[4], st);
+        kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
     }
     if (timed) (void)hipEventRecord(ev[5], st);
     TailArgs ta;
@@ -498,12 +588,12 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
                            hipStream_t st) {
     if (graph_mode == 0 || (graph_mode < 0 && !e->use_graphs)) return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
     const int MT = (int)(round_up(B, 4) / 4);
-    const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT);
+    const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT), fz = e->fused_for(MT);
     timed_valid = false;
     for (auto &g : graphs)
         if (g.fs.slots == fs.slots && g.fs.stride == fs.stride && g.fs.sidx == fs.sidx && g.B == B && g.out == outp &&
             g.out_stride == out_stride && g.oidx == oidx && g.status == st_dev && g.st == st && g.vs == vs &&
-            g.vp == vp) {
+            g.vp == vp && g.fz == fz) {
             g.used = ++tick;
             return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
         }
@@ -525,7 +615,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     const bool inst = HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
     if (!inst) return -1;
-    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, exec, ++tick});
+    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, fz, exec, ++tick});
     return HIP_OK(hipGraphLaunch(exec, st)) ? 0 : -1;
 }
 
@@ -722,6 +812,9 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, const uint8
     for (const auto &v : kVariants) {
         v.prepare_scan();
         v.prepare_pack();
+    }
+    for (const FusedPair &f : kFused) {
+        f.prepare();
     }
     // resident state
     const size_t dbb = rows * cols, ubb = 4 * e->kp * cols;
@@ -950,6 +1043,20 @@ int alto_set_graphs(void *p, int on) {
     if (!on) e->ctx0->drop_graphs();
     e->use_graphs = on != 0;
     return 0;
+}
+
+int alto_set_fusion(void *p, int on) {
+    auto *e = (Engine *)p;
+    if (!e) return -1;
+    const int prev = e->fuse ? 1 : 0;
+    e->fuse = on != 0;
+    return prev;
+}
+
+int alto_fused(const void *p, size_t B) {
+    auto *e = (const Engine *)p;
+    if (!e || !B) return -1;
+    return e->fused_for((int)(round_up(B, 4) / 4)) >= 0 ? 1 : 0;
 }
 
 int alto_variant_count(void) { return kNumVariants; }

@@ -261,8 +261,8 @@ struct TileMap {
     int x, y, z;
     bool valid;
 };
-__device__ __forceinline__ TileMap map_tile(int gx, int gy, int gz) {
-    const int L = blockIdx.x, xcd = L & 7;
+__device__ __forceinline__ TileMap map_tile_at(int L, int gx, int gy, int gz) {
+    const int xcd = L & 7;
     int t = L >> 3;
     const int gxg = (gx + 7) >> 3;
     TileMap m;
@@ -273,6 +273,7 @@ __device__ __forceinline__ TileMap map_tile(int gx, int gy, int gz) {
     m.valid = m.x < gx && m.z < gz;
     return m;
 }
+__device__ __forceinline__ TileMap map_tile(int gx, int gy, int gz) { return map_tile_at(blockIdx.x, gx, gy, gz); }
 
 struct GemmArgs {
     int KC;        // 64-deep k-steps of the contraction
@@ -334,10 +335,10 @@ __device__ __forceinline__ void pack_store(const GemmArgs &g, int rt, int ct, co
 // (x, y, z) computes A tiles [y*MB, y*MB+MB) x B tiles [(4x+w)*NBT, +NBT) over the k-range
 // of split z (at most kFoldSteps steps, so the i32 accumulators cannot overflow).
 template <int MB, int NBT, int U, bool PACK, bool ATOMIC>
-__global__ __launch_bounds__(256) void stream_kernel(const i32x4 *__restrict__ a_src,
-                                                     const i32x4 *__restrict__ b_src, GemmArgs g) {
+__device__ __forceinline__ void stream_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                            const GemmArgs &g, int L) {
     static_assert(!PACK || NBT % 4 == 0, "packing tiles come in groups of four planes");
-    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    const TileMap tm = map_tile_at(L, g.gx, g.gy, g.gz);
     if (!tm.valid) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int bt0 = (tm.x * 4 + wave) * NBT, mt0 = tm.y * MB;
@@ -396,6 +397,11 @@ __global__ __launch_bounds__(256) void stream_kernel(const i32x4 *__restrict__ a
                 if (bt0 + n < g.NT) scan_store<ATOMIC>(g, mt0 + m, bt0 + n, acc[m][n]);
         }
     }
+}
+template <int MB, int NBT, int U, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(256) void stream_kernel(const i32x4 *__restrict__ a_src,
+                                                     const i32x4 *__restrict__ b_src, GemmArgs g) {
+    stream_tile<MB, NBT, U, PACK, ATOMIC>(a_src, b_src, g, blockIdx.x);
 }
 
 // LDS-staged contraction for many queries (matrix-core bound).  Workgroup = WM x WN waves,
@@ -514,8 +520,8 @@ typedef __attribute__((address_space(3))) void lds_void;
 // ring of S stages, so S - 1 k-steps of loads are in flight while the matrix cores work
 // and no VGPRs hold prefetched data.  Needs S * (BM + BN) KiB of dynamic LDS.
 template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1>
-__global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__restrict__ a_src,
-                                                              const i32x4 *__restrict__ b_src, GemmArgs g) {
+__device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                         const GemmArgs &g, int L) {
     constexpr int NW = WM * WN;  // waves: 4 (one per SIMD) or 8 (two per SIMD)
     static_assert(NW == 4 || NW == 8, "four or eight waves");
     static_assert(S >= 2 && S <= 4, "stages");
@@ -527,7 +533,7 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__res
     constexpr int PW = NBLK / NW;
     constexpr int BS = b_kstride<PACK>();
     extern __shared__ i32x4 ring[];  // [S][NBLK][64]
-    const TileMap tm = map_tile(g.gx, g.gy, g.gz);
+    const TileMap tm = map_tile_at(L, g.gx, g.gy, g.gz);
     if (!tm.valid) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave / WN, wn = wave % WN;
@@ -608,6 +614,11 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__res
             }
         }
     }
+}
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1>
+__global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__restrict__ a_src,
+                                                              const i32x4 *__restrict__ b_src, GemmArgs g) {
+    dma_tile<TM, TN, WM, WN, S, PACK, ATOMIC, KPB>(a_src, b_src, g, blockIdx.x);
 }
 
 // dma_kernel with the LDS reads of a k-step overlapped with the matrix work of the step
@@ -715,6 +726,42 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__re
             }
         }
     }
+}
+
+// ---------------------------------------------------- both contractions in one launch --
+
+// Tile shapes as types, so that one kernel can run the tiles of two contractions.
+template <int MB, int NBT, int U, bool PACK, bool ATOMIC>
+struct StreamShape {
+    static constexpr int kThreads = 256;
+    static constexpr size_t kLds = 0;
+    static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
+        stream_tile<MB, NBT, U, PACK, ATOMIC>(a, b, g, L);
+    }
+};
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB>
+struct DmaShape {
+    static constexpr int kThreads = 64 * WM * WN;
+    static constexpr size_t kLds = (size_t)S * (WM * TM + WN * TN) * kBlockBytes;
+    static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
+        dma_tile<TM, TN, WM, WN, S, PACK, ATOMIC, KPB>(a, b, g, L);
+    }
+};
+
+// The scan and the packing contraction of a batch are independent (the tail needs both), so
+// one launch can run them side by side: workgroups [0, scan_blocks) take scan tiles, the rest
+// packing tiles.  Both shapes have the same workgroup size; the dynamic LDS is the larger
+// of the two.  scan_blocks is a multiple of 8, so the XCD-aware tile map of each part sees
+// the same XCD for a workgroup as a launch of its own would.
+template <class Scan, class Pack>
+__global__ __launch_bounds__(Scan::kThreads, 1) void scan_pack_kernel(const i32x4 *__restrict__ qa,
+                                                                      const i32x4 *__restrict__ db, GemmArgs gs,
+                                                                      const i32x4 *__restrict__ ya,
+                                                                      const i32x4 *__restrict__ u, GemmArgs gp,
+                                                                      int scan_blocks) {
+    static_assert(Scan::kThreads == Pack::kThreads, "one workgroup size");
+    if ((int)blockIdx.x < scan_blocks) Scan::tile(qa, db, gs, blockIdx.x);
+    else Pack::tile(ya, u, gp, blockIdx.x - scan_blocks);
 }
 
 // ---------------------------------------------------------------------------- tail --

@@ -240,6 +240,14 @@ class Engine:
         """Graph capture/replay of device-resident batches (alto_set_graphs)."""
         self._lib.alto_set_graphs(self.ptr, 1 if on else 0)
 
+    def set_fusion(self, on: bool) -> None:
+        """One launch for scan and packing where the chosen shape pair allows (default off)."""
+        self._lib.alto_set_fusion(self.ptr, int(on))
+
+    def fused(self, b: int) -> bool:
+        """Whether a batch of b queries runs its two contractions in one launch."""
+        return self._lib.alto_fused(self.ptr, b) == 1
+
     def last_timings(self) -> list:
         ms = (C.c_double * 6)()
         k = self._lib.alto_last_timings(self.ptr, C.addressof(ms), 6)

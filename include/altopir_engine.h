@@ -140,6 +140,18 @@ int alto_context_timings(void *ctx, double *ms, int count);
  * graphs off.  Returns 0, -1 on a null engine. */
 int alto_set_graphs(void *e, int on);
 
+/* One launch for both contractions of a batch (off by default; measured neutral on MI355X:
+ * each contraction alone already fills the chip): when the scan and packing
+ * shapes chosen for a batch size are a pair the library fuses (the built-in policy's pairs),
+ * the packing tiles run in the same launch as the scan tiles (they are independent), after
+ * both limb preparations.  The stage timings then report the pair under "scan" and zero
+ * for packing preparation and packing.  alto_set_fusion returns the previous setting (0 or
+ * 1), -1 on a null engine;
+ * alto_fused returns 1 if a batch of B queries runs fused with the current settings, 0 if
+ * not, -1 on bad arguments. */
+int alto_set_fusion(void *e, int on);
+int alto_fused(const void *e, size_t B);
+
 /* Device time in milliseconds of the stages of the last batch (synchronous entry point, or
  * device entry point with graphs off), waiting for it to finish: {validate, scan limb preparation, scan
  * contraction, packing limb preparation, packing contraction, tail}.  Writes up to

@@ -328,3 +328,29 @@ def test_tuner_records_a_variant_per_bucket_and_keeps_the_bytes(hip_device, smal
         assert fresh._lib.alto_tune_get(fresh.ptr, autotune.PACK, ent["B"]) == names.index(ent["pack"][0])
     fresh.close()
     eng.close()
+
+
+@pytest.mark.parametrize("B", [1, 6, 12, 40, 136])
+def test_scan_and_packing_in_one_launch_give_the_same_bytes(hip_device, small_inputs, B):
+    # the built-in shape pairs run both contractions in one launch (B = 1 and 6 streaming,
+    # 40 the 16x16 ring, 136 the 16x24 ring; 12 has no fused pair): fused, unfused and oracle
+    # answers agree
+    inp = small_inputs
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=1200 + B)
+    eng = Engine.from_inputs(inp)
+    assert not eng.fused(B)  # off by default
+    outs = []
+    for on in (True, False):
+        eng.set_fusion(on)
+        assert eng.fused(B) == (on and B != 12)
+        out = np.zeros((B, inp.response_len), np.uint8)
+        st = np.full(B, -1, np.int32)
+        answered, stage = eng.answer(reqs, out, st)
+        assert answered == B and (st == 0).all()
+        outs.append(out)
+    eng.close()
+    out_o = np.zeros_like(outs[0])
+    st_o = np.full(B, -1, np.int32)
+    assert oc.oracle_answer(inp, reqs, out_o, st_o) == B
+    assert np.array_equal(outs[0], out_o), "fused responses differ from the oracle"
+    assert np.array_equal(outs[1], out_o), "unfused responses differ from the oracle"
