@@ -59,10 +59,13 @@ class DeviceBatch:
             raise RuntimeError(f"alto_answer_device returned {rc} ({_lib.last_error()})")
 
 
-def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict:
-    """Median device time (ms) of the chosen contraction for every variant."""
+def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 0) -> dict:
+    """Mean device time (ms) of the chosen contraction for every variant, each timed over
+    back-to-back batches (no host wait between them, as in a serving or benchmark loop:
+    the clocks a contraction sustains under load differ from those of an isolated launch).
+    reps = 0 picks the count per variant so that about 50 ms of batches run (3 to 20)."""
     lib = _lib.load()
-    stage = 2 if which == SCAN else 4  # index in alto_last_timings
+    stage = 2 if which == SCAN else 4  # index in the stage times
     out = {}
     fusion = lib.alto_set_fusion(engine.ptr, 0)  # each contraction timed in a launch of its own
     saved = lib.alto_tune_get(engine.ptr, which, batch.B)
@@ -71,17 +74,22 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 3) -> dict
         if name.startswith("stream") and mt > 16:
             continue  # register-streaming shapes re-read the database once per 4..32 queries
         lib.alto_tune_set(engine.ptr, which, batch.B, idx)
-        ts = []
-        for _ in range(reps + 1):
+        batch.run()  # first use: instantiation, workspace
+        engine.stage_totals(reset=True)
+        batch.run()
+        n1, t1 = engine.stage_totals(reset=True)
+        step_ms = max(sum(t1) / max(n1, 1), 1e-3)
+        k = reps if reps > 0 else int(min(20, max(3, round(50.0 / step_ms))))
+        for _ in range(k):
             batch.run()
-            ts.append(engine.last_timings()[stage])
-        out[name] = float(np.median(ts[1:]))
+        n, sums = engine.stage_totals(reset=True)
+        out[name] = sums[stage] / n
     lib.alto_tune_set(engine.ptr, which, batch.B, saved)
     lib.alto_set_fusion(engine.ptr, fusion)
     return out
 
 
-def tune(engine, inp, batch_sizes, seed: int = 7, reps: int = 3, log=None) -> dict:
+def tune(engine, inp, batch_sizes, seed: int = 7, reps: int = 0, log=None) -> dict:
     """Times every variant for each batch size and records the fastest in the engine.
     Returns {bucket: {"scan": (name, ms), "pack": (name, ms), "B": B}}."""
     lib = _lib.load()

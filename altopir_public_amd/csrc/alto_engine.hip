@@ -104,27 +104,27 @@ void launch_lds(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
 
 // Allows the dynamic LDS of a dma_kernel instantiation (done once per engine creation,
 // outside any stream capture).
-template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1, bool PRIO = false>
 void prepare_dma() {
     constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
-    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB>,
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB, PRIO>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB>,
+    (void)hipFuncSetAttribute((const void *)dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB, PRIO>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 void prepare_none() {}
 
-template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, int KPB = 1, bool PRIO = false>
 void launch_dma(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     constexpr int NBLK = WM * TM + WN * TN;
     constexpr size_t lds = (size_t)S * NBLK * 1024;
     const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB>), dim3(grid), dim3(64 * WM * WN), lds,
-                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, true, KPB, PRIO>), dim3(grid), dim3(64 * WM * WN),
+                           lds, st, (const i32x4 *)a, (const i32x4 *)b, g);
     else
-        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB>), dim3(grid), dim3(64 * WM * WN), lds,
-                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dma_kernel<TM, TN, WM, WN, S, PACK, false, KPB, PRIO>), dim3(grid), dim3(64 * WM * WN),
+                           lds, st, (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
 template <int TM, int TN, int WM, int WN, int S, bool PACK>
@@ -198,6 +198,11 @@ const Variant kVariants[] = {
      prepare_dma<8, 6, 2, 4, 4, false>, prepare_dma<4, 12, 4, 2, 4, true>},
     {"dma8_m4_n12_s3", launch_dma<8, 6, 2, 4, 3, false>, launch_dma<4, 12, 4, 2, 3, true>,
      prepare_dma<8, 6, 2, 4, 3, false>, prepare_dma<4, 12, 4, 2, 3, true>},
+    // the same shapes with the matrix work of a k-step issued at raised wave priority
+    {"dma8p_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false, 1, true>, launch_dma<4, 8, 4, 2, 4, true, 1, true>,
+     prepare_dma<4, 8, 4, 2, 4, false, 1, true>, prepare_dma<4, 8, 4, 2, 4, true, 1, true>},
+    {"dma8k2p_m8_n6_s4", launch_dma<8, 6, 2, 4, 4, false, 2, true>, launch_dma<4, 8, 4, 2, 4, true, 2, true>,
+     prepare_dma<8, 6, 2, 4, 4, false, 2, true>, prepare_dma<4, 8, 4, 2, 4, true, 2, true>},
     {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
      prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,

@@ -519,7 +519,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 // 16 bytes land at 16 * l of the destination, which is exactly the block layout) through a
 // ring of S stages, so S - 1 k-steps of loads are in flight while the matrix cores work
 // and no VGPRs hold prefetched data.  Needs S * (BM + BN) KiB of dynamic LDS.
-template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1, bool PRIO = false>
 __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
                                          const GemmArgs &g, int L) {
     constexpr int NW = WM * WN;  // waves: 4 (one per SIMD) or 8 (two per SIMD)
@@ -587,6 +587,9 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
             i32x4 b[TN];
 #pragma unroll
             for (int n = 0; n < TN; n++) b[n] = cur[(BM + wn * TN + n) * 64 + lane];
+            // PRIO: a wave that has its operands issues its matrix work ahead of the other
+            // wave of its SIMD (which is still waiting on LDS or at the barrier)
+            if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int m = 0; m < TM; m++) {
                 const i32x4 a = cur[(wm * TM + m) * 64 + lane];
@@ -594,6 +597,7 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
                 for (int n = 0; n < TN; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[m][n], 0, 0, 0);
             }
+            if (PRIO) __builtin_amdgcn_s_setprio(0);
         }
     }
 #pragma unroll
@@ -615,10 +619,10 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
         }
     }
 }
-template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1>
+template <int TM, int TN, int WM, int WN, int S, bool PACK, bool ATOMIC, int KPB = 1, bool PRIO = false>
 __global__ __launch_bounds__(64 * WM * WN, 1) void dma_kernel(const i32x4 *__restrict__ a_src,
                                                               const i32x4 *__restrict__ b_src, GemmArgs g) {
-    dma_tile<TM, TN, WM, WN, S, PACK, ATOMIC, KPB>(a_src, b_src, g, blockIdx.x);
+    dma_tile<TM, TN, WM, WN, S, PACK, ATOMIC, KPB, PRIO>(a_src, b_src, g, blockIdx.x);
 }
 
 // dma_kernel with the LDS reads of a k-step overlapped with the matrix work of the step
