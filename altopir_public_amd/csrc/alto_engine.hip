@@ -198,6 +198,12 @@ const Variant kVariants[] = {
      prepare_dma<8, 6, 2, 4, 4, false>, prepare_dma<4, 12, 4, 2, 4, true>},
     {"dma8_m4_n12_s3", launch_dma<8, 6, 2, 4, 3, false>, launch_dma<4, 12, 4, 2, 3, true>,
      prepare_dma<8, 6, 2, 4, 3, false>, prepare_dma<4, 12, 4, 2, 3, true>},
+    // 32 x 8-tile workgroups: all 128 queries of a batch in one row block, so every database
+    // tile is streamed once (25% more operand bytes per MFMA than 16 x 16)
+    {"dma8_m4_n8_w81_s4", launch_dma<4, 8, 8, 1, 4, false>, launch_dma<4, 8, 8, 1, 4, true>,
+     prepare_dma<4, 8, 8, 1, 4, false>, prepare_dma<4, 8, 8, 1, 4, true>},
+    {"dma8_m4_n8_w81_s3", launch_dma<4, 8, 8, 1, 3, false>, launch_dma<4, 8, 8, 1, 3, true>,
+     prepare_dma<4, 8, 8, 1, 3, false>, prepare_dma<4, 8, 8, 1, 3, true>},
     // the same shapes with the matrix work of a k-step issued at raised wave priority
     {"dma8p_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false, 1, true>, launch_dma<4, 8, 4, 2, 4, true, 1, true>,
      prepare_dma<4, 8, 4, 2, 4, false, 1, true>, prepare_dma<4, 8, 4, 2, 4, true, 1, true>},

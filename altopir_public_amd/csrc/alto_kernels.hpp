@@ -868,13 +868,25 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
                 const uint4 b0 = ld4(bc + base), b1 = ld4(bc + base + 4);
                 s8[0] = b0.x; s8[1] = b0.y; s8[2] = b0.z; s8[3] = b0.w;
                 s8[4] = b1.x; s8[5] = b1.y; s8[6] = b1.z; s8[7] = b1.w;
-                for (int k = 0; k < p.tg; k++) {
-                    const uint4 z0 = ld4(zk + k * n + base), z1 = ld4(zk + k * n + base + 4);
-                    const uint4 t0 = ld4(th + k * n + base), t1 = ld4(th + k * n + base + 4);
-                    s8[0] += mulmod(z0.x, t0.x); s8[1] += mulmod(z0.y, t0.y);
-                    s8[2] += mulmod(z0.z, t0.z); s8[3] += mulmod(z0.w, t0.w);
-                    s8[4] += mulmod(z1.x, t1.x); s8[5] += mulmod(z1.y, t1.y);
-                    s8[6] += mulmod(z1.z, t1.z); s8[7] += mulmod(z1.w, t1.w);
+                // keys four at a time: all their loads are issued before the first product
+                for (int k0 = 0; k0 < p.tg; k0 += 4) {
+                    uint4 z0[4], z1[4], t0[4], t1[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int k = min(k0 + j, p.tg - 1);
+                        z0[j] = ld4(zk + k * n + base);
+                        z1[j] = ld4(zk + k * n + base + 4);
+                        t0[j] = ld4(th + k * n + base);
+                        t1[j] = ld4(th + k * n + base + 4);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (k0 + j >= p.tg) break;
+                        s8[0] += mulmod(z0[j].x, t0[j].x); s8[1] += mulmod(z0[j].y, t0[j].y);
+                        s8[2] += mulmod(z0[j].z, t0[j].z); s8[3] += mulmod(z0[j].w, t0[j].w);
+                        s8[4] += mulmod(z1[j].x, t1[j].x); s8[5] += mulmod(z1[j].y, t1[j].y);
+                        s8[6] += mulmod(z1[j].z, t1[j].z); s8[7] += mulmod(z1[j].w, t1[j].w);
+                    }
                 }
 #pragma unroll
                 for (int m = 0; m < 8; m++) v[m] = red64(s8[m]);

@@ -110,6 +110,8 @@ def main() -> int:
                     help="skip the second pass of the timed steps replayed from captured HIP graphs")
     ap.add_argument("--fusion", action="store_true",
                     help="run scan and packing in one launch where the shape pair allows")
+    ap.add_argument("--scan-variant", default=None, help="force a scan contraction variant (alto_variant_name)")
+    ap.add_argument("--pack-variant", default=None, help="force a packing contraction variant")
     ap.add_argument("--seed", type=int, default=2, help="request seed")
     ap.add_argument("--tune", action="store_true",
                     help="time every tile-shape variant per batch size first and keep the fastest")
@@ -256,6 +258,14 @@ def main() -> int:
         tuned = {str(v["B"]): {"scan": v["scan"][0], "pack": v["pack"][0]} for v in table.values()}
 
     eng.set_fusion(args.fusion)
+    if args.scan_variant or args.pack_variant:
+        from altopir_public_amd import autotune
+
+        names = autotune.variant_names()
+        for b in [B, *args.also_batch]:
+            for which, v in ((autotune.SCAN, args.scan_variant), (autotune.PACK, args.pack_variant)):
+                if v:
+                    eng._lib.alto_tune_set(eng.ptr, which, b, names.index(v))
     main = measure(B, args.steps, args.warmup, seed=args.seed)
     ms_per_step, qps, stage_ms, roof = main["ms_per_step"], main["qps"], main["stage_ms"], main["roofline"]
     slots = main.pop("_slots")
