@@ -18,7 +18,7 @@ SYMBOLS = (
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
     "alto_engine_new_device", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
     "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
-    "alto_context_set_graphs", "alto_set_fusion", "alto_fused",
+    "alto_context_set_graphs", "alto_set_fusion", "alto_fused", "alto_set_graph_timing",
     "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
@@ -89,6 +89,8 @@ def load() -> C.CDLL:
     lib.alto_engine_context.argtypes = [P]
     lib.alto_context_reserve.restype = C.c_int
     lib.alto_context_reserve.argtypes = [P, SZ]
+    lib.alto_set_graph_timing.restype = C.c_int
+    lib.alto_set_graph_timing.argtypes = [P, C.c_int]
     lib.alto_set_fusion.restype = C.c_int
     lib.alto_set_fusion.argtypes = [P, C.c_int]
     lib.alto_fused.restype = C.c_int

@@ -338,6 +338,7 @@ struct Engine {
     int tuned[2][kBuckets];      // [scan, pack][bucket] -> variant, -1 = built-in policy
     bool use_graphs = true;
     bool fuse = false;           // one launch for scan and packing where the shape pair allows
+    bool time_graphs = false;    // graph replays record the stage events too
     Context *ctx0 = nullptr;     // the context of alto_answer / alto_answer_device
 
     Engine() {
@@ -378,6 +379,7 @@ struct Context {
     // scan, pack prep, pack, tail), so that back-to-back batches are timed without the host
     // waiting for each one; a set is folded into the running sums before it is reused.
     static constexpr int kRing = 32;
+    static constexpr int kGraphRing = 4;  // sets cycled by timed graph replays (one graph per set)
     hipEvent_t ring[kRing][7] = {};
     bool ring_pending[kRing] = {};
     int ring_head = -1;          // set of the most recent timed batch
@@ -385,6 +387,8 @@ struct Context {
     double stage_sum[6] = {};    // ms, over the folded sets
     long long stage_count = 0;
     bool timed_valid = false;
+    void record(hipEvent_t x, hipStream_t st) { (void)hipEventRecord(x, st); }
+    bool add_stage_events(hipGraph_t graph, bool fused, hipEvent_t *evs);
     std::vector<int> hstatus;
     // captured batches, replayed when the same batch (pointers, size, stream, kernel
     // variants) comes again
@@ -397,6 +401,7 @@ struct Context {
         int *status;
         hipStream_t st;
         int vs, vp, fz;
+        int ring;  // timing event set recorded by the graph, -1 for none
         hipGraphExec_t exec;
         uint64_t used;
     };
@@ -427,8 +432,8 @@ struct Context {
         return true;
     }
     // Moves to the next event set for a timed batch (folding what it held).
-    hipEvent_t *next_set() {
-        ring_head = (ring_head + 1) % kRing;
+    hipEvent_t *next_set(int span = kRing) {
+        ring_head = (ring_head + 1) % span;
         (void)fold(ring_head);
         ring_pending[ring_head] = true;
         ev = ring[ring_head];
@@ -510,9 +515,9 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
                      hipStream_t st, bool timed, bool record_start) {
     const Engine &E = *e;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
-    if (timed && record_start) (void)hipEventRecord(next_set()[0], st);
+    if (timed && record_start) record(ev[0], st);
     hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, st_dev, (uint32_t)E.kp, qsum, ysum);
-    if (timed) (void)hipEventRecord(ev[1], st);
+    if (timed) record(ev[1], st);
     GemmArgs gs{};  // scan: query limbs x database -> acc
     gs.KC = (int)E.KC;
     gs.MT = (int)MT;
@@ -540,25 +545,25 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
         hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
                            (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
-        if (timed) (void)hipEventRecord(ev[2], st);
+        if (timed) record(ev[2], st);
         kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
         if (timed) {
-            (void)hipEventRecord(ev[3], st);
-            (void)hipEventRecord(ev[4], st);
+            record(ev[3], st);
+            record(ev[4], st);
         }
     } else {
         hipLaunchKernelGGL((limb_prep_kernel<true>), scan_prep_grid, dim3(256), 0, st, fs, (size_t)(2 * E.kp),
                            (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
-        if (timed) (void)hipEventRecord(ev[2], st);
+        if (timed) record(ev[2], st);
         kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
-        if (timed) (void)hipEventRecord(ev[3], st);
+        if (timed) record(ev[3], st);
         hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
                            (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
-        if (timed) (void)hipEventRecord(ev[4], st);
+        if (timed) record(ev[4], st);
         kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
     }
-    if (timed) (void)hipEventRecord(ev[5], st);
+    if (timed) record(ev[5], st);
     TailArgs ta;
     ta.fs = fs;
     ta.status = st_dev;
@@ -583,7 +588,7 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     ta.q1 = (uint32_t)E.q1;
     const size_t lds = (E.n + E.part_words) * 4;
     hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
-    if (timed) (void)hipEventRecord(ev[6], st);
+    if (timed) record(ev[6], st);
     timed_valid = timed;
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
@@ -593,18 +598,73 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     return 0;
 }
 
+// Inserts the seven stage events of a batch into its captured launch chain as event-record
+// nodes, at the positions where an eager batch records them: the chain is validate, scan
+// limb prep, fill, scan, packing limb prep, packing, tail (fused: validate, both limb preps,
+// fill, scan + packing, tail).
+bool Context::add_stage_events(hipGraph_t graph, bool fused, hipEvent_t *evs) {
+    std::vector<hipGraphNode_t> chain;
+    hipGraphNode_t node = nullptr;
+    size_t n = 1;
+    if (!HIP_OK(hipGraphGetRootNodes(graph, &node, &n)) || n != 1) return false;
+    while (true) {
+        chain.push_back(node);
+        size_t nd = 0;
+        if (!HIP_OK(hipGraphNodeGetDependentNodes(node, nullptr, &nd))) return false;
+        if (nd == 0) break;
+        if (nd != 1 || chain.size() > 16) return false;
+        if (!HIP_OK(hipGraphNodeGetDependentNodes(node, &node, &nd))) return false;
+    }
+    const int want = fused ? 6 : 7;
+    if ((int)chain.size() != want) {
+        set_err("graph timing: %d nodes in the captured batch, expected %d", (int)chain.size(), want);
+        return false;
+    }
+    // event k is recorded after chain node after[k] (-1: before the first)
+    static const int after_unfused[7] = {-1, 0, 2, 3, 4, 5, 6}, after_fused[7] = {-1, 0, 3, 4, 4, 4, 5};
+    const int *after = fused ? after_fused : after_unfused;
+    for (size_t i = 0; i + 1 < chain.size(); i++)
+        if (!HIP_OK(hipGraphRemoveDependencies(graph, &chain[i], &chain[i + 1], 1))) return false;
+    hipGraphNode_t prev = nullptr;
+    int k = 0;
+    for (int pos = -1; pos < (int)chain.size(); pos++) {
+        if (pos >= 0) {
+            if (prev && !HIP_OK(hipGraphAddDependencies(graph, &prev, &chain[pos], 1))) return false;
+            prev = chain[pos];
+        }
+        for (; k < 7 && after[k] == pos; k++) {
+            hipGraphNode_t ev_node = nullptr;
+            if (!HIP_OK(hipGraphAddEventRecordNode(&ev_node, graph, prev ? &prev : nullptr, prev ? 1 : 0, evs[k])))
+                return false;
+            prev = ev_node;
+        }
+    }
+    return k == 7;
+}
+
 // The device entry points: eager with stage events when graphs are off, otherwise captured
 // once per distinct batch and replayed.
 int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
                            hipStream_t st) {
-    if (graph_mode == 0 || (graph_mode < 0 && !e->use_graphs)) return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
+    if (graph_mode == 0 || (graph_mode < 0 && !e->use_graphs)) {
+        next_set();
+        return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
+    }
     const int MT = (int)(round_up(B, 4) / 4);
     const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT), fz = e->fused_for(MT);
-    timed_valid = false;
+    // timed replays: the batch is captured once per timing event set, each graph recording
+    // its set's events, and the replays cycle through kGraphRing sets
+    const bool tg = e->time_graphs;
+    int rs = -1;
+    if (tg) {
+        next_set(kGraphRing);
+        rs = ring_head;
+    }
+    timed_valid = tg;
     for (auto &g : graphs)
         if (g.fs.slots == fs.slots && g.fs.stride == fs.stride && g.fs.sidx == fs.sidx && g.B == B && g.out == outp &&
             g.out_stride == out_stride && g.oidx == oidx && g.status == st_dev && g.st == st && g.vs == vs &&
-            g.vp == vp && g.fz == fz) {
+            g.vp == vp && g.fz == fz && g.ring == rs) {
             g.used = ++tick;
             return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
         }
@@ -618,7 +678,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     if (!HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return -1;
     const int rc = enqueue(fs, B, outp, out_stride, oidx, st_dev, st, false);
     const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
-    if (rc || !ended) {
+    if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, ring[rs]))) {
         if (graph) (void)hipGraphDestroy(graph);
         return -1;
     }
@@ -626,7 +686,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     const bool inst = HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
     if (!inst) return -1;
-    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, fz, exec, ++tick});
+    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, fz, rs, exec, ++tick});
     return HIP_OK(hipGraphLaunch(exec, st)) ? 0 : -1;
 }
 
@@ -1053,6 +1113,13 @@ int alto_set_graphs(void *p, int on) {
     if (!e) return -1;
     if (!on) e->ctx0->drop_graphs();
     e->use_graphs = on != 0;
+    return 0;
+}
+
+int alto_set_graph_timing(void *p, int on) {
+    auto *e = (Engine *)p;
+    if (!e) return -1;
+    e->time_graphs = on != 0;
     return 0;
 }
 

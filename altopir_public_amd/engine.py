@@ -240,6 +240,10 @@ class Engine:
         """Graph capture/replay of device-resident batches (alto_set_graphs)."""
         self._lib.alto_set_graphs(self.ptr, 1 if on else 0)
 
+    def set_graph_timing(self, on: bool) -> None:
+        """Graph replays record the stage events too (one captured graph per event set)."""
+        self._lib.alto_set_graph_timing(self.ptr, int(on))
+
     def set_fusion(self, on: bool) -> None:
         """One launch for scan and packing where the chosen shape pair allows (default off)."""
         self._lib.alto_set_fusion(self.ptr, int(on))

@@ -136,9 +136,17 @@ int alto_context_timings(void *ctx, double *ms, int count);
 /* Graph mode of alto_answer_device (on by default): the first batch with a given set of
  * pointers, size, stream and kernel variants is captured into a HIP graph and every later
  * identical batch replays it (at most 64 graphs per engine, least recently used dropped;
- * growing the batch workspace drops them all).  Stage timings are recorded only with
- * graphs off.  Returns 0, -1 on a null engine. */
+ * growing the batch workspace drops them all).  Stage timings are recorded with graphs off,
+ * or with graphs on when alto_set_graph_timing is on.  Returns 0, -1 on a null engine. */
 int alto_set_graphs(void *e, int on);
+
+/* Stage timing of graph replays (off by default): when on, a batch is captured once per
+ * timing event set (4 sets, so up to 4 graphs per distinct batch) with the seven stage
+ * events inserted as event-record nodes, and each replay feeds alto_stage_totals and
+ * alto_last_timings as an eager batch does.  Each event node adds about 8 us to a replay
+ * on MI355X (ROCm 7.2), so this is a diagnostic, not the way to time throughput.  Returns
+ * 0, -1 on a null engine. */
+int alto_set_graph_timing(void *e, int on);
 
 /* One launch for both contractions of a batch (off by default; measured neutral on MI355X:
  * each contraction alone already fills the chip): when the scan and packing

@@ -297,6 +297,50 @@ def test_packed_product_matches_the_oracle(hip_device, rows, cols):
         assert np.array_equal(got, want), fn
 
 
+def test_timed_graph_replays_feed_the_stage_totals(hip_device, small_inputs):
+    # with graph timing on, each replay records its stage events as graph nodes (one captured
+    # graph per event set); every replay is counted, its times are positive and of the same
+    # size as eager launches', and the responses equal the eager ones
+    import torch
+
+    inp = small_inputs
+    B = 6
+    L, rl = inp.request_len, inp.response_len
+    stride = (7 + L + 255) // 256 * 256
+    eng = Engine.from_inputs(inp)
+    host = np.zeros((B, stride), np.uint8)
+    for c, f in enumerate(workload.make_requests(B, inp.rows, inp.kp, seed=93)):
+        host[c, 7:7 + L] = f
+    slots = torch.from_numpy(host).to("cuda")
+    status = torch.zeros(B, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    outs, totals = [], []
+    for graphs in (False, True):
+        out = torch.zeros((B, rl), dtype=torch.uint8, device="cuda")
+        eng.set_graphs(graphs)
+        eng.set_graph_timing(graphs)
+        for _ in range(5):  # captures
+            assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(),
+                                     stream.cuda_stream) == 0
+        stream.synchronize()
+        eng.stage_totals(reset=True)
+        for _ in range(13):
+            assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(),
+                                     stream.cuda_stream) == 0
+        n, ms = eng.stage_totals(reset=True)
+        assert n == 13, (graphs, n)
+        assert all(v > 0 for v in (ms[0], ms[2], ms[4], ms[5])), (graphs, ms)
+        totals.append(ms)
+        outs.append(out.cpu().numpy())
+    eng.set_graph_timing(False)
+    assert int(status.sum()) == 0
+    assert np.array_equal(outs[0], outs[1])
+    # the contractions' times agree within a factor of two between eager and replayed batches
+    for i in (2, 4):
+        assert 0.5 < totals[1][i] / totals[0][i] < 2.0, (i, totals)
+    eng.close()
+
+
 def test_tuner_records_a_variant_per_bucket_and_keeps_the_bytes(hip_device, small_inputs, tmp_path):
     # the tuner times every variant on device-resident batches and records the fastest per
     # batch-size bucket; answers after tuning equal the oracle's, and a saved table applies
