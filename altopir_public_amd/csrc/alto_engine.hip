@@ -842,7 +842,8 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
     });
 }
 
-static void *engine_create(const uint8_t *db_col, bool db_on_device, const uint8_t *db_tiled, size_t ld, size_t rows,
+static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_rows, const uint8_t *db_tiled, size_t ld,
+                           size_t rows,
                            size_t cols, const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,
                            size_t part_len, size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
                            const uint64_t *inv_table, uint64_t q, int threads) {
@@ -893,12 +894,21 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, const uint8
          HIP_OK(hipMalloc(&e->ucorr, cols * 4)) && HIP_OK(hipMalloc(&e->that, e->outputs * e->kp * 4)) &&
          HIP_OK(hipMalloc(&e->wtab, n * 4)) && HIP_OK(hipMalloc(&e->prefix, e->outputs * e->part_words * 4 + 4));
     e->device_bytes = dbb + ubb + cols * 4 + e->outputs * e->kp * 4 + n * 4 + e->outputs * e->part_words * 4;
-    // database: column slices through a staging buffer, tiled on the device
+    // database: column slices (or record slices) through a staging buffer, tiled on the device
     uint8_t *stage = nullptr;
     const size_t stage_cols = std::max<size_t>(16, std::min<size_t>(cols, ((size_t)256 << 20) / rows / 16 * 16));
-    const size_t stage_bytes = std::max(stage_cols * rows, e->kp * cols);
+    const size_t stage_rows = std::max<size_t>(64, std::min<size_t>(rows, ((size_t)256 << 20) / cols / 64 * 64));
+    const size_t stage_bytes = std::max(db_rows ? stage_rows * cols : stage_cols * rows, e->kp * cols);
     ok = ok && HIP_OK(hipMalloc(&stage, stage_bytes));
-    for (size_t c0 = 0; ok && c0 < cols; c0 += stage_cols) {
+    for (size_t r0 = 0; ok && db_rows && r0 < rows; r0 += stage_rows) {
+        const size_t nr = std::min(stage_rows, rows - r0);
+        ok = HIP_OK(hipMemcpy2DAsync(stage, cols, db_col + r0 * ld, ld, cols, nr,
+                                     db_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, setup));
+        hipLaunchKernelGGL(tile_rows_kernel, dim3((unsigned)ceil_div(e->CT, 16), (unsigned)(nr / 64)), dim3(256), 0,
+                           setup, (const uint8_t *)stage, cols, e->db, e->CT, e->KC, r0 / 64);
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(setup));
+    }
+    for (size_t c0 = 0; ok && !db_rows && c0 < cols; c0 += stage_cols) {
         const size_t nc = std::min(stage_cols, cols - c0);
         ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc,
                                      db_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, setup));
@@ -948,7 +958,7 @@ void *alto_engine_new(const uint8_t *db_col, const uint8_t *db_tiled, size_t ld,
                       const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len,
                       size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
                       const uint64_t *inv_table, uint64_t q, int threads) {
-    return engine_create(db_col, false, db_tiled, ld, rows, cols, ub, n, tg, prefix, part_len, row1_off, q1, q1_bits,
+    return engine_create(db_col, false, false, db_tiled, ld, rows, cols, ub, n, tg, prefix, part_len, row1_off, q1, q1_bits,
                          that, inv_table, q, threads);
 }
 
@@ -956,8 +966,20 @@ void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t row
                              size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len, size_t row1_off,
                              uint64_t q1, int q1_bits, const uint64_t *const *that, const uint64_t *inv_table,
                              uint64_t q, int threads) {
-    return engine_create(db_col_device, true, db_col_device, ld, rows, cols, ub, n, tg, prefix, part_len, row1_off,
-                         q1, q1_bits, that, inv_table, q, threads);
+    return engine_create(db_col_device, true, false, db_col_device, ld, rows, cols, ub, n, tg, prefix, part_len,
+                         row1_off, q1, q1_bits, that, inv_table, q, threads);
+}
+
+void *alto_engine_new_records(const uint8_t *records, int on_device, size_t ld, size_t rows, size_t cols,
+                              const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix, size_t part_len,
+                              size_t row1_off, uint64_t q1, int q1_bits, const uint64_t *const *that,
+                              const uint64_t *inv_table, uint64_t q, int threads) {
+    if (ld < cols) {
+        set_err("alto_engine_new_records: record stride below the record size");
+        return nullptr;
+    }
+    return engine_create(records, on_device != 0, true, records, ld, rows, cols, ub, n, tg, prefix, part_len,
+                         row1_off, q1, q1_bits, that, inv_table, q, threads);
 }
 
 void alto_engine_free(void *p) { delete (Engine *)p; }

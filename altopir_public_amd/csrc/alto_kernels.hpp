@@ -85,6 +85,55 @@ __global__ void tile_db_kernel(const uint8_t *__restrict__ src, size_t pitch, in
     *(uint4 *)(dst + ((ct0 + ct) * KC + kc) * kBlockBytes + lane * 16) = f;
 }
 
+// Database ingest from records (row-major: record k = `pitch`-strided row of bytes) into the
+// same blocks as tile_db_kernel.  A workgroup takes 64 records (one k-step) x 256 columns (16
+// column tiles): the records are read row-wise into LDS (rows padded to 65 words, so the
+// column-wise reads of the four 16-record groups of a block fall in different banks) and
+// written as 16 blocks of 1 KiB.  Records kc0 * 64 .. (kc0 + gridDim.y) * 64 of the source.
+__global__ __launch_bounds__(256) void tile_rows_kernel(const uint8_t *__restrict__ src, size_t pitch,
+                                                        int8_t *__restrict__ dst, size_t CT, size_t KC,
+                                                        size_t kc0) {
+    __shared__ uint32_t t[64][65];
+    const size_t kcl = blockIdx.y, ct0 = (size_t)blockIdx.x * 16;
+    const int nct = (int)min((size_t)16, CT - ct0);
+    {
+        const int r = threadIdx.x >> 2, q = threadIdx.x & 3;  // record r, columns 64q .. 64q + 63
+        const uint8_t *row = src + (kcl * 64 + r) * pitch + ct0 * 16;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int c16 = q * 4 + e;  // 16-column chunk = column tile ct0 + c16
+            if (c16 < nct) {
+                const uint4 v = *(const uint4 *)(row + c16 * 16);
+                t[r][c16 * 4] = v.x;
+                t[r][c16 * 4 + 1] = v.y;
+                t[r][c16 * 4 + 2] = v.z;
+                t[r][c16 * 4 + 3] = v.w;
+            }
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int b = it * 4 + (threadIdx.x >> 6);  // column tile ct0 + b
+        if (b >= nct) continue;
+        // block[16 lane + j] = byte (16 b + lane % 16) of record 16 (lane / 16) + j
+        const int col = b * 16 + (lane & 15), r0 = 16 * (lane >> 4);
+        uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint8_t *rowb = (const uint8_t *)&t[r0 + 4 * i + j][0];
+                x |= (uint32_t)rowb[col] << (8 * j);
+            }
+            w[i] = x ^ 0x80808080u;
+        }
+        *(uint4 *)(dst + ((ct0 + b) * KC + kc0 + kcl) * kBlockBytes + lane * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // One byte plane j of the packing matrix, in the reference's AMX B-tile layout
 // (src/amx_kernels.cpp:39-41, 331-336: plane region of KP * cols bytes, 32-column pairs,
 // row r of a 1 KiB tile = 16 columns x 4 consecutive k), already offset by -128 ->

@@ -171,6 +171,25 @@ class Engine:
                    inp.q1_bits, inp.that, inp.inv_table, Q, threads)
 
     @classmethod
+    def from_records(cls, records_ptr: int, on_device: bool, ld: int, inp, threads: int = 8) -> "Engine":
+        """Engine over a database given as records (row-major, record k at records_ptr + k * ld,
+        host or device memory), transposed into the engine's layout on the device; the other
+        setup inputs come from workload.EngineInputs."""
+        self = cls.__new__(cls)
+        self._lib = _lib.load()
+        prefixes = [np.ascontiguousarray(p, dtype=np.uint8) for p in inp.prefixes]
+        that = [np.ascontiguousarray(t, dtype=np.uint64) for t in inp.that]
+        inv_table = np.ascontiguousarray(inp.inv_table, dtype=np.uint64)
+        pp = (C.c_void_p * len(prefixes))(*[_ptr(p) for p in prefixes])
+        tp = (C.c_void_p * len(that))(*[_ptr(t) for t in that])
+        self.ptr = self._lib.alto_engine_new_records(records_ptr, int(on_device), ld, inp.rows, inp.cols, _ptr(inp.ub),
+                                                     inp.n, inp.tg, C.addressof(pp), inp.part_len, inp.row1_off,
+                                                     inp.q1, inp.q1_bits, C.addressof(tp), _ptr(inv_table), Q, threads)
+        assert self.ptr, f"unsupported engine configuration ({_lib.last_error()})"
+        self.rows, self.cols = inp.rows, inp.cols
+        return self
+
+    @classmethod
     def from_device_db(cls, db_ptr: int, inp, threads: int = 8) -> "Engine":
         """Engine over a column-major database already in device memory (db_ptr, ld =
         inp.ld); the other setup inputs come from workload.EngineInputs."""

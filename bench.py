@@ -112,6 +112,8 @@ def main() -> int:
                     help="run scan and packing in one launch where the shape pair allows")
     ap.add_argument("--scan-variant", default=None, help="force a scan contraction variant (alto_variant_name)")
     ap.add_argument("--pack-variant", default=None, help="force a packing contraction variant")
+    ap.add_argument("--ingest-records", action="store_true",
+                    help="build the engine from the device-resident bytes taken as records (row-major)")
     ap.add_argument("--seed", type=int, default=2, help="request seed")
     ap.add_argument("--tune", action="store_true",
                     help="time every tile-shape variant per batch size first and keep the fastest")
@@ -151,10 +153,16 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"[bench] inputs {R} x {C} B generated in {time.time() - t0:.1f} s")
     t0 = time.time()
-    eng = Engine.from_device_db(d_db.data_ptr(), inp, threads=args.cpu_threads)
+    if args.ingest_records:
+        # the same bytes taken as records (row-major), transposed into the engine's layout on
+        # the device: a different database of the same distribution
+        eng = Engine.from_records(d_db.data_ptr(), True, C, inp, threads=args.cpu_threads)
+    else:
+        eng = Engine.from_device_db(d_db.data_ptr(), inp, threads=args.cpu_threads)
     del d_db
     torch.cuda.empty_cache()
-    log(f"[bench] engine set up in {time.time() - t0:.1f} s, {eng.device_bytes() / 2**30:.2f} GiB resident")
+    log(f"[bench] engine set up in {time.time() - t0:.2f} s from {'records' if args.ingest_records else 'columns'}, "
+        f"{eng.device_bytes() / 2**30:.2f} GiB resident")
     L, rl = inp.request_len, inp.response_len
     stride = round_up(7 + L, 256)
     KP = tg * n

@@ -185,6 +185,16 @@ void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t row
                              const uint64_t *const *that, const uint64_t *inv_table, uint64_t q,
                              int threads);
 
+/* alto_engine_new with the database given as records (row-major, as a database file holds
+ * them: record k, `cols` bytes, at records + k * ld, ld >= cols), in host memory
+ * (on_device = 0) or device memory; the records are moved to the device in slices and
+ * transposed into the engine's layout there (the reference transposes on the host,
+ * src/server.rs:251-366).  The other arguments and checks are those of alto_engine_new. */
+void *alto_engine_new_records(const uint8_t *records, int on_device, size_t ld, size_t rows, size_t cols,
+                              const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,
+                              size_t part_len, size_t row1_off, uint64_t q1, int q1_bits,
+                              const uint64_t *const *that, const uint64_t *inv_table, uint64_t q, int threads);
+
 /* Kernel variants of the two contractions (scan and packing) and the per-engine tuning
  * table that picks one per batch-size bucket (buckets: ceil(log2(ceil(B / 4))))).
  * which: 0 = scan, 1 = packing; idx -1 restores the built-in choice.  Every variant

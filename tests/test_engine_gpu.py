@@ -398,3 +398,33 @@ def test_scan_and_packing_in_one_launch_give_the_same_bytes(hip_device, small_in
     assert oc.oracle_answer(inp, reqs, out_o, st_o) == B
     assert np.array_equal(outs[0], out_o), "fused responses differ from the oracle"
     assert np.array_equal(outs[1], out_o), "unfused responses differ from the oracle"
+
+
+@pytest.mark.parametrize("rows,cols,n,ld_pad,on_device", [(2048, 4096, 2048, 0, False), (2048, 4096, 2048, 7, True),
+                                                         (192, 352, 32, 32, False), (128, 96, 32, 0, True)])
+def test_engine_from_records_matches_the_oracle(hip_device, rows, cols, n, ld_pad, on_device):
+    # the database given record by record (row-major, as a file holds it) is transposed into the
+    # engine's layout on the device: answers equal the column-major engine's and the oracle's
+    import torch
+
+    inp = workload.make_inputs(rows, cols, n=n, tg=4, seed=rows + cols)
+    ld = cols + ld_pad
+    rec = np.zeros((rows, ld), np.uint8)
+    rec[:, :cols] = inp.db_record_major()
+    B = 5
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=77)
+    if on_device:
+        d_rec = torch.from_numpy(rec).to("cuda")
+        eng = Engine.from_records(d_rec.data_ptr(), True, ld, inp)
+        del d_rec
+    else:
+        eng = Engine.from_records(rec.ctypes.data, False, ld, inp)
+    out = np.zeros((B, inp.response_len), np.uint8)
+    st = np.full(B, -1, np.int32)
+    answered, stage = eng.answer(reqs, out, st)
+    eng.close()
+    out_o = np.zeros_like(out)
+    st_o = np.full(B, -1, np.int32)
+    r_o = oc.oracle_answer(inp, reqs, out_o, st_o)
+    assert answered == r_o == B and np.array_equal(st, st_o)
+    assert np.array_equal(out, out_o), "responses of the record-ingested engine differ from the oracle"
