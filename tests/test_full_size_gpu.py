@@ -31,8 +31,10 @@ def part_inputs(inp, o, db_cols_host):
                                  inv_table=inp.inv_table, part_len=inp.part_len, row1_off=inp.row1_off)
 
 
-@pytest.mark.parametrize("rows,parts", [(32768, None), (2097152, (0, 15))])
-def test_full_size_parts_match_the_oracle(hip_device, rows, parts):
+@pytest.mark.parametrize("rows,parts,batch", [(32768, None, 2), (262144, (0, 7), 136), (2097152, (0, 15), 2)])
+def test_full_size_parts_match_the_oracle(hip_device, rows, parts, batch):
+    # 1 GiB, all parts; 8 GiB with an uneven batch past 128 queries (the 16 x 24-tile shape);
+    # 64 GiB (32 record splits of the scan)
     import torch
 
     cols, n, tg = 32768, 2048, 4
@@ -40,7 +42,7 @@ def test_full_size_parts_match_the_oracle(hip_device, rows, parts):
     d_db = synthetic.uniform_bytes_device(0, rows * cols)
     torch.cuda.synchronize()
     eng = Engine.from_device_db(d_db.data_ptr(), inp)
-    reqs = workload.make_requests(2, rows, inp.kp, seed=2)
+    reqs = workload.make_requests(batch, rows, inp.kp, seed=2)
     rng = np.random.default_rng(9)
     reqs.append(workload.corrupt(reqs[0], rng, inp.kp))  # a malformed frame at full size
     B, rl = len(reqs), inp.response_len
