@@ -24,10 +24,18 @@ def variant_names() -> list[str]:
     return [lib.alto_variant_name(i).decode() for i in range(lib.alto_variant_count())]
 
 
+def bucket_top(b: int) -> int:
+    """Largest A-tile count (4 queries a tile) of tuning bucket b: 1, 2, 3, 4, 6, 8, 12, ..."""
+    if b < 2:
+        return b + 1
+    return 2 << (b >> 1) if b & 1 else 3 << ((b >> 1) - 1)
+
+
 def bucket(B: int) -> int:
+    """Tuning bucket of a batch of B queries (the engine's bucket_of on ceil(B / 4))."""
     mt = (B + 3) // 4
     b = 0
-    while b + 1 < 16 and (1 << b) < mt:
+    while b + 1 < 30 and bucket_top(b) < mt:
         b += 1
     return b
 

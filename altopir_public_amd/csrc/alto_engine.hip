@@ -204,6 +204,12 @@ const Variant kVariants[] = {
      prepare_dma<4, 8, 8, 1, 4, false>, prepare_dma<4, 8, 8, 1, 4, true>},
     {"dma8_m4_n8_w81_s3", launch_dma<4, 8, 8, 1, 3, false>, launch_dma<4, 8, 8, 1, 3, true>,
      prepare_dma<4, 8, 8, 1, 3, false>, prepare_dma<4, 8, 8, 1, 3, true>},
+    // 12-row-tile workgroups (48 queries): less padding for batches of 33-48 queries; scan
+    // only (their column tiles do not come in fours), packing runs dma8_m4_n8_s4
+    {"dma8_m3_n6_s4", launch_dma<3, 6, 4, 2, 4, false>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<3, 6, 4, 2, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
+    {"dma8_m3_n10_s4", launch_dma<3, 10, 4, 2, 4, false>, launch_dma<4, 8, 4, 2, 4, true>,
+     prepare_dma<3, 10, 4, 2, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
     // the same shapes with the matrix work of a k-step issued at raised wave priority
     {"dma8p_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false, 1, true>, launch_dma<4, 8, 4, 2, 4, true, 1, true>,
      prepare_dma<4, 8, 4, 2, 4, false, 1, true>, prepare_dma<4, 8, 4, 2, 4, true, 1, true>},
@@ -306,12 +312,15 @@ int fused_index(int vs, int vp) {
     return -1;
 }
 
-// Batch-size buckets of the tuning table: A tiles (4 queries each) 1, 2, 3-4, 5-8, ...,
-// i.e. bucket = ceil(log2(MT)), up to 2^15 tiles.
-constexpr int kBuckets = 16;
+// Batch-size buckets of the tuning table, in half octaves of A tiles (4 queries each):
+// bucket b holds MT up to bucket_top(b) = 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, ... (2^k and
+// 3 * 2^k tiles), so that the 12-, 16-, 24- and 32-tile row blocks of the ring kernels each
+// end a bucket; the last bucket ends at 2^15 tiles and takes every larger batch.
+constexpr int kBuckets = 30;
+constexpr long bucket_top(int b) { return b < 2 ? b + 1 : (b & 1) ? 2L << (b >> 1) : 3L << ((b >> 1) - 1); }
 int bucket_of(int MT) {
     int b = 0;
-    while (b + 1 < kBuckets && (1 << b) < MT) b++;
+    while (b + 1 < kBuckets && bucket_top(b) < MT) b++;
     return b;
 }
 

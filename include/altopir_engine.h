@@ -196,7 +196,8 @@ void *alto_engine_new_records(const uint8_t *records, int on_device, size_t ld, 
                               const uint64_t *const *that, const uint64_t *inv_table, uint64_t q, int threads);
 
 /* Kernel variants of the two contractions (scan and packing) and the per-engine tuning
- * table that picks one per batch-size bucket (buckets: ceil(log2(ceil(B / 4))))).
+ * table that picks one per batch-size bucket (half octaves of MT = ceil(B / 4): bucket b
+ * ends at MT = 1, 2, 3, 4, 6, 8, 12, 16, 24, ... for b = 0, 1, 2, ...; 30 buckets).
  * which: 0 = scan, 1 = packing; idx -1 restores the built-in choice.  Every variant
  * computes the same bytes; they differ only in tile shapes and operand staging. */
 int alto_variant_count(void);
