@@ -210,6 +210,13 @@ const Variant kVariants[] = {
      prepare_dma<3, 6, 4, 2, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
     {"dma8_m3_n10_s4", launch_dma<3, 10, 4, 2, 4, false>, launch_dma<4, 8, 4, 2, 4, true>,
      prepare_dma<3, 10, 4, 2, 4, false>, prepare_dma<4, 8, 4, 2, 4, true>},
+    // 8-row-tile workgroups of eight waves (32 queries), for batches of 17-32 queries
+    {"dma8_m2_n8_s4", launch_dma<2, 8, 4, 2, 4, false>, launch_dma<2, 8, 4, 2, 4, true>,
+     prepare_dma<2, 8, 4, 2, 4, false>, prepare_dma<2, 8, 4, 2, 4, true>},
+    {"dma8_m2_n12_s4", launch_dma<2, 12, 4, 2, 4, false>, launch_dma<2, 12, 4, 2, 4, true>,
+     prepare_dma<2, 12, 4, 2, 4, false>, prepare_dma<2, 12, 4, 2, 4, true>},
+    {"dma8_m4_n8_w24_s4", launch_dma<4, 8, 2, 4, 4, false>, launch_dma<4, 8, 2, 4, 4, true>,
+     prepare_dma<4, 8, 2, 4, 4, false>, prepare_dma<4, 8, 2, 4, 4, true>},
     // the same shapes with the matrix work of a k-step issued at raised wave priority
     {"dma8p_m4_n8_s4", launch_dma<4, 8, 4, 2, 4, false, 1, true>, launch_dma<4, 8, 4, 2, 4, true, 1, true>,
      prepare_dma<4, 8, 4, 2, 4, false, 1, true>, prepare_dma<4, 8, 4, 2, 4, true, 1, true>},
@@ -285,18 +292,23 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
 // from sweeps over batch sizes 1-2048 on 1 and 8 GiB databases: streaming kernels up to 8
-// queries, the 4-wave ring for 9-16 queries (scan), the 8-wave 16 x 16-tile ring up to 128
-// queries and the 16 x 24-tile ring with two k-steps per barrier above (it needs enough
-// query rows to fill the chip with its wider tiles).
+// queries (and for packing up to 16); the scan of 9-32 queries on 8 x 32-tile workgroups
+// (each streamed query row is used against twice the database columns of the square tile)
+// and its packing on 8 x 16 tiles; the scan of 33-48 queries on 12-row tiles; the 8-wave
+// 16 x 16-tile ring up to 128 queries; above it the scan on 32 x 12 tiles with two k-steps
+// per barrier (it needs enough query rows to fill the chip with its wider tiles).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
-                     stream4 = variant_index("stream_m4_n4_u2"), dma4 = variant_index("dma_m4_n8_s4"),
-                     dma8 = variant_index("dma8_m4_n8_s4"), wide = variant_index("dma8k2_m8_n6_s4");
+                     stream4 = variant_index("stream_m4_n4_u2"), dma8 = variant_index("dma8_m4_n8_s4"),
+                     rows8 = variant_index("dma8_m4_n8_w24_s4"), rows8p = variant_index("dma8_m2_n8_s4"),
+                     rows12 = variant_index("dma8_m3_n10_s4"), wide = variant_index("dma8k2_m8_n6_s4");
     if (MT == 1) return stream1;
     if (MT == 2) return stream2;
-    if (MT <= 4) return which == 0 ? dma4 : stream4;
-    if (MT <= 32 || which == 1) return dma8;
+    if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : dma8;
+    if (MT <= 8) return rows8;
+    if (MT <= 12) return rows12;
+    if (MT <= 32) return dma8;
     return wide;
 }
 int variant_index(const char *name) {
@@ -752,7 +764,9 @@ int packed_product(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t r
         g.NT = (int)CT;
         g.C = (int)R16;
         g.out = acc;
-        kVariants[default_variant(0, (int)MT)].scan(g, qa, db, st);
+        const Variant &v = kVariants[default_variant(0, (int)MT)];
+        v.prepare_scan();  // the ring kernels' LDS size, which engine set-up otherwise sets
+        v.scan(g, qa, db, st);
         hipLaunchKernelGGL(packed_finish_kernel, dim3((unsigned)((rows * bc + 255) / 256)), dim3(256), 0, st, dout,
                            (const uint32_t *)acc, (const uint32_t *)qsum, rows, bc, R16);
         ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(out, dout, rows * bc * 4, hipMemcpyDeviceToHost, st)) &&
