@@ -285,26 +285,28 @@ struct FusedPair {
 const FusedPair kFused[] = {
     {"stream_m1_n4_u4", "stream_m1_n4_u4", launch_fused_stream<1, 4, 4, 1, 4, 4>, prepare_none},
     {"stream_m2_n4_u4", "stream_m2_n4_u4", launch_fused_stream<2, 4, 4, 2, 4, 4>, prepare_none},
+    {"stream_m2_n4_u4", "stream_m1_n4_u4", launch_fused_stream<2, 4, 4, 1, 4, 4>, prepare_none},
     {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
     {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
-// from sweeps over batch sizes 1-2048 on 1 and 8 GiB databases: streaming kernels up to 8
-// queries (and for packing up to 16); the scan of 9-32 queries on 8 x 32-tile workgroups
-// (each streamed query row is used against twice the database columns of the square tile)
-// and its packing on 8 x 16 tiles; the scan of 33-48 queries on 12-row tiles; the 8-wave
-// 16 x 16-tile ring up to 128 queries; above it the scan on 32 x 12 tiles with two k-steps
-// per barrier (it needs enough query rows to fill the chip with its wider tiles).
+// from sweeps over batch sizes 1-2048 on 1-64 GiB databases: streaming kernels up to 8
+// queries, and for packing up to 16 (the scan of 1-4 queries runs the 8-query streaming
+// shape, whose loads issue in a better order: 2% faster at 1-4 GiB, even at 64 GiB); the
+// scan of 9-32 queries on 8 x 32-tile workgroups (each streamed query row meets twice the
+// database columns of the square tile) and its packing on 8 x 16 tiles; the scan of 33-48
+// queries on 12-row tiles; the 8-wave 16 x 16-tile ring up to 128 queries; above it the
+// scan on 32 x 12 tiles with two k-steps per barrier (it needs enough query rows to fill
+// the chip with its wider tiles).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), dma8 = variant_index("dma8_m4_n8_s4"),
                      rows8 = variant_index("dma8_m4_n8_w24_s4"), rows8p = variant_index("dma8_m2_n8_s4"),
                      rows12 = variant_index("dma8_m3_n10_s4"), wide = variant_index("dma8k2_m8_n6_s4");
-    if (MT == 1) return stream1;
-    if (MT == 2) return stream2;
+    if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : dma8;
     if (MT <= 8) return rows8;
     if (MT <= 12) return rows12;
