@@ -59,13 +59,15 @@ int choose_splits(int KC, long long tiles, int target, int min_steps) {
 }
 
 // Workgroup grid of one contraction (sets g.gx, g.gy, g.gz; returns the workgroup count):
-// streaming shapes split the record range until about 2048 workgroups, ring shapes until
-// 256 (one per CU); the packing contraction is never split (KP fits one i32 range).
-template <int MB, int NBT>
+// streaming shapes split the record range until about WGS workgroups (1024 unless a variant
+// says otherwise: measured best or level at 1-64 GiB, 7% faster than 2048 at 8 GiB), ring
+// shapes until 256 (one per CU); the packing contraction is never split (KP fits one i32
+// range).
+template <int MB, int NBT, int WGS = 1024>
 int plan_stream(GemmArgs &g, bool pack) {
     g.gx = ceil_div(g.NT, 4 * NBT);
     g.gy = ceil_div(g.MT, MB);
-    g.gz = pack ? 1 : choose_splits(g.KC, (long long)g.gx * g.gy, 2048, 16);
+    g.gz = pack ? 1 : choose_splits(g.KC, (long long)g.gx * g.gy, WGS, 16);
     return 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
 }
 template <int TM, int TN, int WM, int WN>
@@ -76,9 +78,9 @@ int plan_ring(GemmArgs &g, bool pack) {
     return 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
 }
 
-template <int MB, int NBT, int U, bool PACK>
+template <int MB, int NBT, int U, bool PACK, int WGS = 1024>
 void launch_stream(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
-    const int grid = plan_stream<MB, NBT>(g, PACK);
+    const int grid = plan_stream<MB, NBT, WGS>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((stream_kernel<MB, NBT, U, PACK, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
                            (const i32x4 *)b, g);
@@ -165,6 +167,11 @@ struct Variant {
 const Variant kVariants[] = {
     {"stream_m1_n4_u4", launch_stream<1, 4, 4, false>, launch_stream<1, 4, 4, true>, prepare_none, prepare_none},
     {"stream_m2_n4_u4", launch_stream<2, 4, 4, false>, launch_stream<2, 4, 4, true>, prepare_none, prepare_none},
+    // the same with the record range split for about 2048 or 4096 workgroups
+    {"stream_m2_n4_u4_z2k", launch_stream<2, 4, 4, false, 2048>, launch_stream<2, 4, 4, true>, prepare_none,
+     prepare_none},
+    {"stream_m2_n4_u4_z4k", launch_stream<2, 4, 4, false, 4096>, launch_stream<2, 4, 4, true>, prepare_none,
+     prepare_none},
     {"stream_m4_n4_u2", launch_stream<4, 4, 2, false>, launch_stream<4, 4, 2, true>, prepare_none, prepare_none},
     {"stream_m8_n4_u2", launch_stream<8, 4, 2, false>, launch_stream<8, 4, 2, true>, prepare_none, prepare_none},
     {"lds_m8_n8", launch_lds<8, 8, 2, 2, false>, launch_lds<8, 8, 2, 2, true>, prepare_none, prepare_none},
