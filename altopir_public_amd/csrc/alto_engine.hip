@@ -616,7 +616,7 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     ta.row1_off = (long long)E.row1_off;
     ta.q1_bits = E.q1_bits;
     ta.q1 = (uint32_t)E.q1;
-    const size_t lds = (E.n + E.part_words) * 4;
+    const size_t lds = tail_lds_bytes(E.n, E.part_words);
     hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
     if (timed) record(ev[6], st);
     timed_valid = timed;
@@ -906,7 +906,7 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
     e->KC = rows / 64; e->KCp = e->kp / 64; e->CT = cols / 16;
     e->k128e = (uint32_t)((128ull * kE) % kQ);
     e->ycorr0 = (uint32_t)((kQ - host_mulmod(host_mulmod(e->k128e, e->k128e), e->kp % kQ)) % kQ);
-    const size_t lds = (n + e->part_words) * 4;
+    const size_t lds = tail_lds_bytes(n, e->part_words);
     e->ctx0 = new Context;
     bool ok = e->ctx0->init(e);
     hipStream_t setup = e->ctx0->stream;
