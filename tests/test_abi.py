@@ -132,3 +132,21 @@ def test_synthetic_record_follows_the_reference_generator():
         v, state = synthetic.mix(state)
         want += int(v % 65535).to_bytes(2, "little")
     assert bytes(synthetic.record(5, 16)) == bytes(want)
+
+
+def test_variant_table_and_tuning_buckets():
+    # the variant names are static data (no device needed); the tuning buckets are half
+    # octaves of MT = ceil(B / 4) whose ends fall on the 8-, 12-, 16- and 32-tile row blocks
+    from altopir_public_amd import autotune
+
+    names = autotune.variant_names()
+    assert len(names) == len(set(names)) and _lib.load().alto_variant_name(len(names)) is None
+    for must in ("stream_m1_n4_u4", "stream_m2_n4_u4", "dma8_m4_n8_s4", "dma8k2_m8_n6_s4", "dma8_m4_n8_w24_s4",
+                 "dma8_m2_n8_s4", "dma8_m3_n10_s4"):
+        assert must in names
+    tops = [autotune.bucket_top(b) for b in range(30)]
+    assert tops[:10] == [1, 2, 3, 4, 6, 8, 12, 16, 24, 32] and tops[-1] == 1 << 15
+    assert all(x < y for x, y in zip(tops, tops[1:]))
+    for b, top in enumerate(tops):
+        assert autotune.bucket(4 * top) == b and autotune.bucket(4 * top + 1) == min(b + 1, 29)
+    assert autotune.bucket(1) == 0 and autotune.bucket(10**7) == 29
