@@ -305,7 +305,7 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 // scan of 9-32 queries on 8 x 32-tile workgroups (each streamed query row meets twice the
 // database columns of the square tile) and its packing on 8 x 16 tiles; the scan of 33-48
 // queries on 12-row tiles; the 8-wave 16 x 16-tile ring up to 128 queries; above it the
-// scan on 32 x 12 tiles with two k-steps per barrier (it needs enough query rows to fill
+// scan on 16 x 24 tiles with two k-steps per barrier (it needs enough query rows to fill
 // the chip with its wider tiles).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
