@@ -405,20 +405,27 @@ struct Context {
     uint8_t *slots = nullptr, *out = nullptr;
     int *status = nullptr;
     hipStream_t stream = nullptr;
-    // Timing events of eager batches: a ring of event sets (start, validate, scan prep,
-    // scan, pack prep, pack, tail), so that back-to-back batches are timed without the host
-    // waiting for each one; a set is folded into the running sums before it is reused.
+    // Timing events of eager batches: a ring of event sets, event i closing stage i - 1 of
+    // (validate, scan prep, scan, pack prep, pack, tail) after event 0 at the start, so that
+    // back-to-back batches are timed without the host waiting for each one; a set is folded
+    // into the running sums before it is reused.  A batch records only the events of its
+    // launch boundaries (ring_mask: bit i = event i recorded); the time between two recorded
+    // events counts for the later stage and the skipped stages read zero.  Both limb
+    // preparations run in one launch (under "scan prep"), so event 4 is never recorded; with
+    // the one-launch contractions neither is event 5 (the pair reads as "scan").
     static constexpr int kRing = 32;
     static constexpr int kGraphRing = 4;  // sets cycled by timed graph replays (one graph per set)
+    static constexpr uint8_t kMaskSplit = 0x6f, kMaskFused = 0x4f;
     hipEvent_t ring[kRing][7] = {};
     bool ring_pending[kRing] = {};
+    uint8_t ring_mask[kRing] = {};
     int ring_head = -1;          // set of the most recent timed batch
     hipEvent_t *ev = nullptr;    // = ring[ring_head]
     double stage_sum[6] = {};    // ms, over the folded sets
     long long stage_count = 0;
     bool timed_valid = false;
     void record(hipEvent_t x, hipStream_t st) { (void)hipEventRecord(x, st); }
-    bool add_stage_events(hipGraph_t graph, bool fused, hipEvent_t *evs);
+    bool add_stage_events(hipGraph_t graph, bool fused, int set);
     std::vector<int> hstatus;
     // captured batches, replayed when the same batch (pointers, size, stream, kernel
     // variants) comes again
@@ -452,13 +459,23 @@ struct Context {
     bool fold(int k) {
         if (!ring_pending[k]) return true;
         ring_pending[k] = false;
-        if (!HIP_OK(hipEventSynchronize(ring[k][6]))) return false;
-        for (int i = 0; i < 6; i++) {
-            float f = 0;
-            (void)hipEventElapsedTime(&f, ring[k][i], ring[k][i + 1]);
-            stage_sum[i] += f;
-        }
+        double t[6];
+        if (!stage_times(k, t)) return false;
+        for (int i = 0; i < 6; i++) stage_sum[i] += t[i];
         stage_count++;
+        return true;
+    }
+    // The six stage times (ms) of event set k, waiting for its last event.
+    bool stage_times(int k, double *t) {
+        if (!HIP_OK(hipEventSynchronize(ring[k][6]))) return false;
+        for (int i = 1, from = 0; i < 7; i++) {
+            t[i - 1] = 0;
+            if (!(ring_mask[k] >> i & 1)) continue;
+            float f = 0;
+            (void)hipEventElapsedTime(&f, ring[k][from], ring[k][i]);
+            t[i - 1] = f;
+            from = i;
+        }
         return true;
     }
     // Moves to the next event set for a timed batch (folding what it held).
@@ -537,10 +554,6 @@ Engine::~Engine() {
         if (p) (void)hipFree(p);
 }
 
-__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
-}
-
 int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
                      hipStream_t st, bool timed, bool record_start) {
     const Engine &E = *e;
@@ -564,36 +577,41 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     gp.ysum = ysum;
     gp.k128e = E.k128e;
     gp.ycorr0 = E.ycorr0;
-    const dim3 scan_prep_grid((unsigned)((E.KC * 4 + 255) / 256), (unsigned)Bp),
-        pack_prep_grid((unsigned)((E.KCp * 4 + 255) / 256), (unsigned)Bp);
+    // one launch prepares both limb sets and clears the scan accumulators
+    PrepArgs pa;
+    pa.fs = fs;
+    pa.status = st_dev;
+    pa.B = (int)B;
+    pa.scan_off = 2 * E.kp;
+    pa.KC = E.KC;
+    pa.KCp = E.KCp;
+    pa.qa = qa;
+    pa.ya = ya;
+    pa.qsum = qsum;
+    pa.ysum = ysum;
+    pa.acc = (uint4 *)acc;
+    pa.acc4 = Bp * E.cols / 4;
+    pa.sx = (unsigned)((E.KC * 4 + 255) / 256);
+    pa.sy = (unsigned)Bp;
+    pa.px = (unsigned)((E.KCp * 4 + 255) / 256);
+    pa.py = (unsigned)Bp;
+    pa.nfill = (unsigned)std::min<size_t>(1024, (pa.acc4 + 1023) / 1024);
+    const unsigned prep_blocks = pa.sx * pa.sy + pa.px * pa.py + pa.nfill;
+    hipLaunchKernelGGL(prep_kernel, dim3(prep_blocks), dim3(256), 0, st, pa);
+    if (timed) record(ev[2], st);
     const int fz = E.fused_for((int)MT);
     if (fz >= 0) {
-        // both limb preparations, then one launch for both contractions; the stage times
-        // show the pair under "scan" (packing preparation and packing read as zero)
-        hipLaunchKernelGGL((limb_prep_kernel<true>), scan_prep_grid, dim3(256), 0, st, fs, (size_t)(2 * E.kp),
-                           (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
-        hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
-                           (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
-        if (timed) record(ev[2], st);
+        // one launch for both contractions; the stage times show the pair under "scan"
+        // (packing preparation and packing read as zero)
         kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
-        if (timed) {
-            record(ev[3], st);
-            record(ev[4], st);
-        }
+        if (timed) record(ev[3], st);
     } else {
-        hipLaunchKernelGGL((limb_prep_kernel<true>), scan_prep_grid, dim3(256), 0, st, fs, (size_t)(2 * E.kp),
-                           (const int *)st_dev, (int)B, E.KC, qa, qsum, ysum);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, st, acc, Bp * E.cols, 0u);
-        if (timed) record(ev[2], st);
         kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
         if (timed) record(ev[3], st);
-        hipLaunchKernelGGL((limb_prep_kernel<false>), pack_prep_grid, dim3(256), 0, st, fs, (size_t)0,
-                           (const int *)st_dev, (int)B, E.KCp, ya, qsum, ysum);
-        if (timed) record(ev[4], st);
         kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
+        if (timed) record(ev[5], st);
     }
-    if (timed) record(ev[5], st);
+    if (timed) ring_mask[ring_head] = fz >= 0 ? kMaskFused : kMaskSplit;
     TailArgs ta;
     ta.fs = fs;
     ta.status = st_dev;
@@ -628,11 +646,11 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     return 0;
 }
 
-// Inserts the seven stage events of a batch into its captured launch chain as event-record
-// nodes, at the positions where an eager batch records them: the chain is validate, scan
-// limb prep, fill, scan, packing limb prep, packing, tail (fused: validate, both limb preps,
-// fill, scan + packing, tail).
-bool Context::add_stage_events(hipGraph_t graph, bool fused, hipEvent_t *evs) {
+// Inserts the stage events of a batch into its captured launch chain as event-record nodes,
+// at the positions where an eager batch records them: the chain is validate, prep, scan,
+// packing, tail (fused: validate, prep, scan + packing, tail).
+bool Context::add_stage_events(hipGraph_t graph, bool fused, int set) {
+    hipEvent_t *evs = ring[set];
     std::vector<hipGraphNode_t> chain;
     hipGraphNode_t node = nullptr;
     size_t n = 1;
@@ -645,31 +663,34 @@ bool Context::add_stage_events(hipGraph_t graph, bool fused, hipEvent_t *evs) {
         if (nd != 1 || chain.size() > 16) return false;
         if (!HIP_OK(hipGraphNodeGetDependentNodes(node, &node, &nd))) return false;
     }
-    const int want = fused ? 6 : 7;
+    const int want = fused ? 4 : 5;
     if ((int)chain.size() != want) {
         set_err("graph timing: %d nodes in the captured batch, expected %d", (int)chain.size(), want);
         return false;
     }
-    // event k is recorded after chain node after[k] (-1: before the first)
-    static const int after_unfused[7] = {-1, 0, 2, 3, 4, 5, 6}, after_fused[7] = {-1, 0, 3, 4, 4, 4, 5};
-    const int *after = fused ? after_fused : after_unfused;
+    // event k is recorded after chain node after[k] (-1: before the first, -2: not recorded)
+    static const int after_split[7] = {-1, 0, 1, 2, -2, 3, 4}, after_fused[7] = {-1, 0, 1, 2, -2, -2, 3};
+    const int *after = fused ? after_fused : after_split;
+    ring_mask[set] = fused ? kMaskFused : kMaskSplit;
     for (size_t i = 0; i + 1 < chain.size(); i++)
         if (!HIP_OK(hipGraphRemoveDependencies(graph, &chain[i], &chain[i + 1], 1))) return false;
     hipGraphNode_t prev = nullptr;
-    int k = 0;
+    int recorded = 0;
     for (int pos = -1; pos < (int)chain.size(); pos++) {
         if (pos >= 0) {
             if (prev && !HIP_OK(hipGraphAddDependencies(graph, &prev, &chain[pos], 1))) return false;
             prev = chain[pos];
         }
-        for (; k < 7 && after[k] == pos; k++) {
+        for (int k = 0; k < 7; k++) {
+            if (after[k] != pos) continue;
             hipGraphNode_t ev_node = nullptr;
             if (!HIP_OK(hipGraphAddEventRecordNode(&ev_node, graph, prev ? &prev : nullptr, prev ? 1 : 0, evs[k])))
                 return false;
             prev = ev_node;
+            recorded++;
         }
     }
-    return k == 7;
+    return recorded == __builtin_popcount(ring_mask[set]);
 }
 
 // The device entry points: eager with stage events when graphs are off, otherwise captured
@@ -696,6 +717,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
             g.out_stride == out_stride && g.oidx == oidx && g.status == st_dev && g.st == st && g.vs == vs &&
             g.vp == vp && g.fz == fz && g.ring == rs) {
             g.used = ++tick;
+            if (rs >= 0) ring_mask[rs] = g.fz >= 0 ? kMaskFused : kMaskSplit;
             return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
         }
     if (graphs.size() >= kMaxGraphs) {
@@ -708,7 +730,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     if (!HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return -1;
     const int rc = enqueue(fs, B, outp, out_stride, oidx, st_dev, st, false);
     const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
-    if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, ring[rs]))) {
+    if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, rs))) {
         if (graph) (void)hipGraphDestroy(graph);
         return -1;
     }
@@ -1151,13 +1173,13 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     // the download is timed as part of the tail stage
     if (!HIP_OK(hipEventRecord(ctx->ev[6], st)) || !HIP_OK(hipStreamSynchronize(st))) return -1;
     if (stage_ms) {
-        // {validate, scan, pack, tail} as in src/amx_kernels.cpp:613
-        const int from[4] = {0, 1, 3, 5}, to[4] = {1, 3, 5, 6};
-        for (int i = 0; i < 4; i++) {
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, ctx->ev[from[i]], ctx->ev[to[i]]);
-            stage_ms[i] = ms;
-        }
+        // {validate, scan, pack, tail} as in src/amx_kernels.cpp:613, from the six stages
+        double t[6];
+        if (!ctx->stage_times(ctx->ring_head, t)) return -1;
+        stage_ms[0] = t[0];
+        stage_ms[1] = t[1] + t[2];
+        stage_ms[2] = t[3] + t[4];
+        stage_ms[3] = t[5];
     }
     return answered;
 }
@@ -1228,14 +1250,11 @@ int alto_stage_totals(void *p, double *ms, int count, int reset) {
 
 int alto_context_timings(void *c, double *ms, int count) {
     auto *ctx = (Context *)c;
-    if (!ctx || !ms || count <= 0 || !ctx->timed_valid) return 0;
-    if (!HIP_OK(hipEventSynchronize(ctx->ev[6]))) return -1;
+    if (!ctx || !ms || count <= 0 || !ctx->timed_valid || ctx->ring_head < 0) return 0;
+    double t[6];
+    if (!ctx->stage_times(ctx->ring_head, t)) return -1;
     const int k = std::min(count, 6);
-    for (int i = 0; i < k; i++) {
-        float f = 0;
-        (void)hipEventElapsedTime(&f, ctx->ev[i], ctx->ev[i + 1]);
-        ms[i] = f;
-    }
+    for (int i = 0; i < k; i++) ms[i] = t[i];
     return k;
 }
 

@@ -255,14 +255,15 @@ __device__ __forceinline__ T block_sum(T v, T *red) {
 // SCAN: signed digits of q + 0x80808080 and sum(q) mod 2^32 into qsum.
 // PACK: bytes of y offset by -128 and the exact sum(y) into ysum.
 // grid = (ceil(KC * 4 / 256), Bp); clients c >= B or with status != 0 get zero rows.
+// Block (bx, by) of that grid; limb_prep_kernel runs the grid, prep_kernel runs both limb
+// preparations and the accumulator clear in one launch.
 template <bool SCAN>
-__global__ __launch_bounds__(256) void limb_prep_kernel(FrameSet fs, size_t word_off, const int *__restrict__ status, int B,
-                                                        size_t KC, int8_t *__restrict__ dst,
-                                                        uint32_t *__restrict__ qsum,
-                                                        unsigned long long *__restrict__ ysum) {
+__device__ __forceinline__ void limb_prep_block(FrameSet fs, size_t word_off, const int *__restrict__ status, int B,
+                                                size_t KC, int8_t *__restrict__ dst, uint32_t *__restrict__ qsum,
+                                                unsigned long long *__restrict__ ysum, unsigned bx, unsigned by) {
     __shared__ unsigned long long red[8];
-    const int c = blockIdx.y;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)by;
+    const size_t p = (size_t)bx * blockDim.x + threadIdx.x;
     const size_t kc = p >> 2;
     const int g = (int)(p & 3);
     const bool ok = c < B && status[c] == 0;
@@ -299,6 +300,45 @@ __global__ __launch_bounds__(256) void limb_prep_kernel(FrameSet fs, size_t word
         if (SCAN) atomicAdd(&qsum[c], (uint32_t)s);
         else atomicAdd(&ysum[c], s);
     }
+}
+template <bool SCAN>
+__global__ __launch_bounds__(256) void limb_prep_kernel(FrameSet fs, size_t word_off, const int *__restrict__ status, int B,
+                                                        size_t KC, int8_t *__restrict__ dst,
+                                                        uint32_t *__restrict__ qsum,
+                                                        unsigned long long *__restrict__ ysum) {
+    limb_prep_block<SCAN>(fs, word_off, status, B, KC, dst, qsum, ysum, blockIdx.x, blockIdx.y);
+}
+
+// One launch for the per-batch preparation after validation: blocks [0, sx * sy) prepare
+// the query limbs (limb_prep_block<true>, grid sx x sy), the next px * py the key limbs
+// (limb_prep_block<false>), the last nfill clear the scan accumulators (acc4 16-byte
+// words).  The three parts are independent, so they run side by side.
+struct PrepArgs {
+    FrameSet fs;
+    const int *status;
+    int B;
+    size_t scan_off, KC, KCp;
+    int8_t *qa, *ya;
+    uint32_t *qsum;
+    unsigned long long *ysum;
+    uint4 *acc;
+    size_t acc4;
+    unsigned sx, sy, px, py, nfill;
+};
+__global__ __launch_bounds__(256) void prep_kernel(PrepArgs p) {
+    unsigned b = blockIdx.x;
+    if (b < p.sx * p.sy) {
+        limb_prep_block<true>(p.fs, p.scan_off, p.status, p.B, p.KC, p.qa, p.qsum, p.ysum, b % p.sx, b / p.sx);
+        return;
+    }
+    b -= p.sx * p.sy;
+    if (b < p.px * p.py) {
+        limb_prep_block<false>(p.fs, 0, p.status, p.B, p.KCp, p.ya, p.qsum, p.ysum, b % p.px, b / p.px);
+        return;
+    }
+    b -= p.px * p.py;
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    for (size_t i = (size_t)b * blockDim.x + threadIdx.x; i < p.acc4; i += (size_t)p.nfill * blockDim.x) p.acc[i] = z;
 }
 
 // ------------------------------------------------------------------- contractions ----

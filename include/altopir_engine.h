@@ -141,8 +141,8 @@ int alto_context_timings(void *ctx, double *ms, int count);
 int alto_set_graphs(void *e, int on);
 
 /* Stage timing of graph replays (off by default): when on, a batch is captured once per
- * timing event set (4 sets, so up to 4 graphs per distinct batch) with the seven stage
- * events inserted as event-record nodes, and each replay feeds alto_stage_totals and
+ * timing event set (4 sets, so up to 4 graphs per distinct batch) with the stage events
+ * inserted as event-record nodes, and each replay feeds alto_stage_totals and
  * alto_last_timings as an eager batch does.  Each event node adds about 8 us to a replay
  * on MI355X (ROCm 7.2), so this is a diagnostic, not the way to time throughput.  Returns
  * 0, -1 on a null engine. */
@@ -153,7 +153,7 @@ int alto_set_graph_timing(void *e, int on);
  * shapes chosen for a batch size are a pair the library fuses (the built-in policy's pairs),
  * the packing tiles run in the same launch as the scan tiles (they are independent), after
  * both limb preparations.  The stage timings then report the pair under "scan" and zero
- * for packing preparation and packing.  alto_set_fusion returns the previous setting (0 or
+ * for packing.  alto_set_fusion returns the previous setting (0 or
  * 1), -1 on a null engine;
  * alto_fused returns 1 if a batch of B queries runs fused with the current settings, 0 if
  * not, -1 on bad arguments. */
@@ -161,8 +161,9 @@ int alto_set_fusion(void *e, int on);
 int alto_fused(const void *e, size_t B);
 
 /* Device time in milliseconds of the stages of the last batch (synchronous entry point, or
- * device entry point with graphs off), waiting for it to finish: {validate, scan limb preparation, scan
- * contraction, packing limb preparation, packing contraction, tail}.  Writes up to
+ * device entry point with graphs off), waiting for it to finish: {validate, limb preparation
+ * (query and key limbs, one launch), scan contraction, packing limb preparation (always 0:
+ * it runs in the launch before), packing contraction, tail}.  Writes up to
  * `count` values and returns how many were written (0 if no batch ran yet). */
 int alto_last_timings(void *e, double *ms, int count);
 
