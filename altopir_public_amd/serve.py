@@ -95,7 +95,8 @@ class Server:
 
     def set_upload(self, mode: int) -> None:
         """0: copy each frame into HBM at submit; 1: kernels read the pinned slots over PCIe;
-        -1: 1 for frames up to 512 KiB, else 0 (the default)."""
+        2: copy a batch's frames into HBM when it is dispatched (a few large copies);
+        -1: 2 for frames up to 512 KiB, else 0 (the default)."""
         if self.lib.alto_server_set_upload(self.ptr, mode) != 0:
             raise RuntimeError("alto_server_set_upload failed (requests pending or bad mode)")
 

@@ -124,9 +124,9 @@ def main() -> int:
     ap.add_argument("--serve-window", type=int, default=6, help="outstanding requests, in batches")
     ap.add_argument("--serve-graphs", type=int, default=-1, choices=[-1, 0, 1],
                     help="graph mode of the served form's contexts (-1 follow the engine)")
-    ap.add_argument("--serve-upload", type=int, default=-1, choices=[-1, 0, 1],
+    ap.add_argument("--serve-upload", type=int, default=-1, choices=[-1, 0, 1, 2],
                     help="served form: 0 copy frames into HBM at submit, 1 kernels read the pinned slots, "
-                         "-1 by frame size")
+                         "2 copy a batch's frames into HBM at dispatch, -1 the server's default")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -264,9 +264,10 @@ int alto_server_replay(void *srv, int db, const uint8_t *const *frames, const si
 int alto_server_set_graphs(void *srv, int mode);
 /* How frames reach the device: 0 each frame is copied into the device arena at submit, on
  * the database's upload stream; 1 the batch's kernels read the frames straight from the
- * pinned slots over PCIe (no per-frame copy); -1 (the default) 1 for databases whose frames
- * are at most 512 KiB, 0 above.  Only while no request is pending or in flight; returns 0,
- * -1 otherwise or on bad arguments. */
+ * pinned slots over PCIe (no per-frame copy); 2 a batch's frames are copied into the device
+ * arena when it is dispatched, one copy per run of consecutive slots, on the batch's stream;
+ * -1 (the default) 2 for databases whose frames are at most 512 KiB, 0 above.  Only while
+ * no request is pending or in flight; returns 0, -1 otherwise or on bad arguments. */
 int alto_server_set_upload(void *srv, int mode);
 
 /* Number of HBM bytes the engine holds for its resident state (database, packing

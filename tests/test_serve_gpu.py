@@ -30,9 +30,10 @@ def oracle_responses(inp, frames):
     return out, st
 
 
-@pytest.mark.parametrize("upload", [0, 1, -1])
+@pytest.mark.parametrize("upload", [0, 1, 2, -1])
 def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, upload):
-    # upload 0: frames copied into HBM at submit; 1: the kernels read the pinned slots; -1: by size
+    # upload 0: frames copied into HBM at submit; 1: the kernels read the pinned slots; 2: a
+    # batch's frames copied into HBM at dispatch; -1: the server's default
     inps = list(two_dbs)
     engines = [Engine.from_inputs(i) for i in inps]
     srv = Server(engines, max_batch=16, slots=64, contexts=2)
