@@ -451,7 +451,10 @@ struct Context {
         e = eng;
         bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto &set : ring)
-            for (auto &x : set) ok = ok && HIP_OK(hipEventCreate(&x));
+            // device-scope release: the stage events only timestamp the stream (nothing reads
+            // device memory on the host through them), so recording one need not write the L2
+            // back to memory the way a system-scope release does
+            for (auto &x : set) ok = ok && HIP_OK(hipEventCreateWithFlags(&x, hipEventReleaseToDevice));
         ev = ring[0];
         return ok;
     }
