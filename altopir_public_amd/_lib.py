@@ -23,7 +23,7 @@ SYMBOLS = (
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
     "alto_server_get_stats", "alto_server_error", "alto_server_free", "alto_server_replay", "alto_server_set_graphs",
-    "alto_server_set_upload",
+    "alto_server_set_upload", "alto_server_upload_mode",
 )
 
 _lib = None

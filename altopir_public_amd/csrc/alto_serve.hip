@@ -437,6 +437,12 @@ int alto_server_set_upload(void *p, int mode) {
     return 0;
 }
 
+int alto_server_upload_mode(void *p, int db) {
+    auto *s = (Server *)p;
+    if (!s || db < 0 || db >= (int)s->dbs.size()) return -1;
+    return s->dbs[db]->upload_mode;
+}
+
 int alto_server_set_graphs(void *p, int mode) {
     auto *s = (Server *)p;
     if (!s || mode < -1 || mode > 1) return -1;

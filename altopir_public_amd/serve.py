@@ -56,6 +56,8 @@ def _bind(lib):
     lib.alto_server_free.argtypes = [P]
     lib.alto_server_set_upload.restype = C.c_int
     lib.alto_server_set_upload.argtypes = [P, C.c_int]
+    lib.alto_server_upload_mode.restype = C.c_int
+    lib.alto_server_upload_mode.argtypes = [P, C.c_int]
     lib.alto_server_set_graphs.restype = C.c_int
     lib.alto_server_set_graphs.argtypes = [P, C.c_int]
     lib.alto_server_replay.restype = C.c_int
@@ -99,6 +101,10 @@ class Server:
         -1: 2 for frames up to 512 KiB, else 0 (the default)."""
         if self.lib.alto_server_set_upload(self.ptr, mode) != 0:
             raise RuntimeError("alto_server_set_upload failed (requests pending or bad mode)")
+
+    def upload_mode(self, db: int = 0) -> int:
+        """The upload mode in force for database db (0, 1 or 2)."""
+        return self.lib.alto_server_upload_mode(self.ptr, db)
 
     def contexts(self, db: int = 0) -> int:
         return self.lib.alto_server_contexts(self.ptr, db)

@@ -321,9 +321,8 @@ def main() -> int:
         served["form"] = (f"alto_server_* loop driven by alto_server_replay: {args.serve_threads} native submitter "
                           f"threads copy host frames into pinned slots, up to {win} requests outstanding")
         served["contexts"] = srv.contexts(0)
-        mode = args.serve_upload if args.serve_upload >= 0 else (2 if L <= 512 << 10 else 0)
         served["upload"] = {0: "frame copied into HBM at submit", 1: "kernels read the pinned slots",
-                            2: "batch copied into HBM at dispatch"}[mode]
+                            2: "batch copied into HBM at dispatch"}[srv.upload_mode(0)]
         srv.close()
         log(f"[bench] served form: {served}")
 

@@ -269,6 +269,8 @@ int alto_server_set_graphs(void *srv, int mode);
  * -1 (the default) 2 for databases whose frames are at most 512 KiB, 0 above.  Only while
  * no request is pending or in flight; returns 0, -1 otherwise or on bad arguments. */
 int alto_server_set_upload(void *srv, int mode);
+/* The upload mode in force for database db (0, 1 or 2 as above), -1 on bad arguments. */
+int alto_server_upload_mode(void *srv, int db);
 
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */

@@ -38,6 +38,8 @@ def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, uploa
     engines = [Engine.from_inputs(i) for i in inps]
     srv = Server(engines, max_batch=16, slots=64, contexts=2)
     srv.set_upload(upload)
+    # frames of these databases are small: the default copies them per batch at dispatch
+    assert [srv.upload_mode(d) for d in range(2)] == [upload if upload >= 0 else 2] * 2
     rng = np.random.default_rng(5)
     frames, dbs = [], []
     for k in range(40):
