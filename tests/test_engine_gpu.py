@@ -110,6 +110,18 @@ def test_multi_block_and_chunked_batches_match_the_oracle(hip_device, B):
     assert_same(g, o, f"B={B}")
 
 
+@pytest.mark.parametrize("B", [3, 9, 24, 33, 48, 49, 129])
+def test_built_in_shapes_at_every_bucket_edge_match_the_oracle(hip_device, B):
+    # the built-in shape choice changes at 4, 8, 32 and 48 queries (streaming, 8-row ring,
+    # 12-row ring, 16-row ring) and at 128 (16 x 24 tiles); each side of those edges, with the
+    # default packing shape of the same batch size
+    inp = _block_inputs()
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=300 + B)
+    g, o = run_both(inp, reqs)
+    assert g[0] == B
+    assert_same(g, o, f"B={B}")
+
+
 _BLOCK = None
 
 
