@@ -927,10 +927,11 @@ __device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *(const uint4 *
 // that stay inside the group (phase 0 only the first 1-3 stages); groups are exchanged
 // through LDS between phases, so n = 2048 needs three barriers instead of eleven.  Phase 0
 // takes its group straight from the pointwise product and the last phase feeds the switch
-// directly.  The twiddles and the part's scan accumulators are copied into LDS at the start,
-// next to the part's prefix, so that the later phases wait on LDS rather than on L2.
-// LDS: a[n] | twiddles[n] | accumulators[n] | part[part_words] (tail_lds_bytes).
-__host__ __device__ constexpr size_t tail_lds_bytes(size_t n, size_t part_words) { return (3 * n + part_words) * 4; }
+// directly.  The twiddles are copied into LDS at the start, next to the part's prefix, so
+// that the later phases wait on LDS rather than on L2; the last phase loads its group's scan
+// accumulators before its butterflies, so that their latency hides behind them.
+// LDS: a[n] | twiddles[n] | part[part_words] (tail_lds_bytes).
+__host__ __device__ constexpr size_t tail_lds_bytes(size_t n, size_t part_words) { return (2 * n + part_words) * 4; }
 
 __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     extern __shared__ uint32_t lds[];
@@ -939,8 +940,7 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     const int n = p.n, logn = p.logn, groups = n >> 3;
     uint32_t *a = lds;
     uint32_t *tw = lds + n;
-    uint32_t *accl = lds + 2 * n;
-    uint32_t *part = lds + 3 * n;
+    uint32_t *part = lds + 2 * n;
     const size_t col0 = (size_t)o * n;
     const uint32_t *pre = (const uint32_t *)(p.prefix + (size_t)o * p.part_words * 4);
     for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) part[w] = pre[w];
@@ -949,13 +949,8 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     const uint32_t *bc = p.beta + (size_t)c * p.C + col0;
     const uint32_t *accc = p.acc + (size_t)c * p.C + col0;
     const uint32_t qs = p.qsum[c] << 7;
-    // n is a power of two of at least 16 and the rows of acc start on 16-byte boundaries
-    for (int w = threadIdx.x; w < (n >> 2); w += blockDim.x) {
-        ((uint4 *)tw)[w] = ld4(p.wtab + 4 * w);
-        uint4 x = ld4(accc + 4 * w);
-        x.x += qs; x.y += qs; x.z += qs; x.w += qs;
-        ((uint4 *)accl)[w] = x;
-    }
+    // n is a power of two of at least 16
+    for (int w = threadIdx.x; w < (n >> 2); w += blockDim.x) ((uint4 *)tw)[w] = ld4(p.wtab + 4 * w);
     // phase 0 takes the r0 = logn - 3 (nph - 1) first stages (1..3) on contiguous groups, so
     // that every later phase has exactly three stages and its groups (stride 2^ls, 8
     // elements) stay inside the n points
@@ -997,6 +992,11 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
 #pragma unroll
                 for (int m = 0; m < 8; m++) v[m] = a[base + (m << ls)];
             }
+            uint32_t accv[8];
+            if (ph + 1 == nph) {
+#pragma unroll
+                for (int m = 0; m < 8; m++) accv[m] = accc[base + (m << ls)] + qs;
+            }
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 if (q >= nst) break;
@@ -1018,8 +1018,7 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
 #pragma unroll
                 for (int m = 0; m < 8; m++) {
                     const int z = base + (m << ls);
-                    const uint32_t acc = ph == 0 ? accc[z] + qs : accl[z];
-                    const uint32_t inter = (uint32_t)(((uint64_t)acc * kQ + (1ull << 31)) >> 32);
+                    const uint32_t inter = (uint32_t)(((uint64_t)accv[m] * kQ + (1ull << 31)) >> 32);
                     const uint32_t r = rescale_q1(addmod(inter, v[m]), p.q1);
                     const unsigned long long bit = (unsigned long long)p.row1_off + (unsigned long long)z * p.q1_bits;
                     const unsigned long long byte0 = bit >> 3;
