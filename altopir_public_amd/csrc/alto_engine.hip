@@ -562,7 +562,8 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     const Engine &E = *e;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
     if (timed && record_start) record(ev[0], st);
-    hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, st_dev, (uint32_t)E.kp, qsum, ysum);
+    hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(E.kp)), dim3(256), 0, st, fs, st_dev,
+                       (uint32_t)E.kp, qsum, ysum);
     if (timed) record(ev[1], st);
     GemmArgs gs{};  // scan: query limbs x database -> acc
     gs.KC = (int)E.KC;
@@ -1153,7 +1154,7 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     if (!ok) return -1;
     if (!e->tail_ok) {
         // validation only: the reference fails the whole batch with -3 once any request is valid
-        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B), dim3(256), 0, st, fs, ctx->status, (uint32_t)e->kp,
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(e->kp)), dim3(256), 0, st, fs, ctx->status, (uint32_t)e->kp,
                            ctx->qsum, ctx->ysum);
         if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
             !HIP_OK(hipStreamSynchronize(st)))

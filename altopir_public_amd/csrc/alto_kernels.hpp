@@ -202,18 +202,25 @@ __device__ __forceinline__ uint32_t ld32_bytes(const uint8_t *p) {
     return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
 }
 
-// Request validation (src/amx_kernels.cpp:571-581): one workgroup per frame.  Frames with
-// a nonzero status on entry (rejected by the host: null or wrong length) keep it.
-// Also clears the per-query sums that the prep kernels accumulate.
+// Request validation (src/amx_kernels.cpp:571-581): grid (B, S), workgroup (c, s) checks
+// slice s of frame c's 2 KP key words.  Frames with a nonzero status on entry (rejected by
+// the host: null or wrong length) keep it.  Every workgroup of a frame reads the same
+// header, so a bad header gives status 2 from all of them and no key is checked; a key >= Q
+// turns a 0 into 3 (compare-and-swap, so a status set on entry stands).  A workgroup that
+// starts after another one's 3 returns at once.  Workgroup (c, 0) also clears the per-query
+// sums that the prep kernel accumulates.
 __global__ __launch_bounds__(256) void validate_kernel(FrameSet fs, int *__restrict__ status, uint32_t kp,
                                                        uint32_t *__restrict__ qsum,
                                                        unsigned long long *__restrict__ ysum) {
     __shared__ int st;
     const size_t c = blockIdx.x;
+    const unsigned s = blockIdx.y, S = gridDim.y;
     if (threadIdx.x == 0) {
-        qsum[c] = 0;
-        ysum[c] = 0;
-        st = status[c];
+        if (s == 0) {
+            qsum[c] = 0;
+            ysum[c] = 0;
+        }
+        st = __hip_atomic_load(&status[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (st != 0) return;
@@ -223,13 +230,19 @@ __global__ __launch_bounds__(256) void validate_kernel(FrameSet fs, int *__restr
         return;
     }
     const uint4 *keys = (const uint4 *)(f + 9);
+    const uint32_t nk = kp / 2, per = (nk + S - 1) / S, k0 = s * per, k1 = min(nk, k0 + per);
     int bad = 0;
-    for (uint32_t i = threadIdx.x; i < kp / 2; i += blockDim.x) {
+    for (uint32_t i = k0 + threadIdx.x; i < k1; i += blockDim.x) {
         const uint4 v = keys[i];
         bad |= (v.x >= kQ32) | (v.y >= kQ32) | (v.z >= kQ32) | (v.w >= kQ32);
     }
     bad = __syncthreads_or(bad);
-    if (threadIdx.x == 0) status[c] = bad ? 3 : 0;
+    if (threadIdx.x == 0 && bad) atomicCAS(&status[c], 0, 3);
+}
+// Slices per frame of validate_kernel: about one 16-byte key load per thread.
+inline unsigned validate_slices(size_t kp) {
+    const size_t s = kp / 2 / 256;
+    return (unsigned)(s < 1 ? 1 : s > 16 ? 16 : s);
 }
 
 // ---------------------------------------------------------- per batch: limb prep ----
