@@ -263,9 +263,11 @@ class Engine:
         """Graph replays record the stage events too (one captured graph per event set)."""
         self._lib.alto_set_graph_timing(self.ptr, int(on))
 
-    def set_fusion(self, on: bool) -> None:
-        """One launch for scan and packing where the chosen shape pair allows (default off)."""
-        self._lib.alto_set_fusion(self.ptr, int(on))
+    def set_fusion(self, on) -> None:
+        """One launch for scan and packing where the chosen shape pair allows: True / 1 for
+        every such pair, False / 0 never, -1 the built-in choice (batches of 1-4 queries)."""
+        if self._lib.alto_set_fusion(self.ptr, int(on)) < -1:
+            raise ValueError(on)
 
     def fused(self, b: int) -> bool:
         """Whether a batch of b queries runs its two contractions in one launch."""
