@@ -381,11 +381,14 @@ struct Engine {
         return v >= 0 ? v : default_variant(which, MT);
     }
     // kFused index used for a batch of MT A tiles, or -1 for two launches
-    // Built-in: fused only for batches of one A tile (1-4 queries), where the one launch of the
-    // two streaming contractions measured 4% faster than two (1 GiB: 0.325 against 0.339 ms);
-    // at two A tiles it was 8% slower and from 16 A tiles on level.
+    // Built-in: fused only for batches of one A tile (1-4 queries) on databases of at most
+    // 4 KP records, whose scan streams no more than twice the bytes of the packing matrix:
+    // there one launch of the two streaming contractions measured 4% faster than two (1 GiB:
+    // 0.325 against 0.339 ms); at 8 and 64 GiB it was 2% and 6% slower (the packing tiles
+    // then interleave with a much longer scan), at two A tiles 8% slower, from 16 A tiles on
+    // level.
     int fused_for(int MT) const {
-        if (fuse == 0 || (fuse < 0 && MT != 1)) return -1;
+        if (fuse == 0 || (fuse < 0 && (MT != 1 || rows > 4 * kp))) return -1;
         return fused_index(variant_for(0, MT), variant_for(1, MT));
     }
     size_t request_len() const { return 9 + 4 * (2 * kp + rows); }

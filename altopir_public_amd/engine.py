@@ -265,7 +265,8 @@ class Engine:
 
     def set_fusion(self, on) -> None:
         """One launch for scan and packing where the chosen shape pair allows: True / 1 for
-        every such pair, False / 0 never, -1 the built-in choice (batches of 1-4 queries)."""
+        every such pair, False / 0 never, -1 the built-in choice (batches of 1-4 queries on
+        databases of at most 4 KP records)."""
         if self._lib.alto_set_fusion(self.ptr, int(on)) < -1:
             raise ValueError(on)
 

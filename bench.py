@@ -110,7 +110,7 @@ def main() -> int:
                     help="skip the second pass of the timed steps replayed from captured HIP graphs")
     ap.add_argument("--fusion", type=int, default=-1, choices=[-1, 0, 1],
                     help="scan and packing in one launch: 1 wherever the shape pair allows, 0 never, "
-                         "-1 the engine's built-in choice (batches of 1-4 queries)")
+                         "-1 the engine's built-in choice (batches of 1-4 queries on databases up to 4 KP records)")
     ap.add_argument("--scan-variant", default=None, help="force a scan contraction variant (alto_variant_name)")
     ap.add_argument("--pack-variant", default=None, help="force a packing contraction variant")
     ap.add_argument("--ingest-records", action="store_true",

@@ -152,9 +152,10 @@ int alto_set_graph_timing(void *e, int on);
  * a batch size are a pair the library fuses (the built-in policy's pairs), the packing
  * tiles run in the same launch as the scan tiles (they are independent), after both limb
  * preparations.  The stage timings then report the pair under "scan" and zero for packing.
- * on: 1 fuse every such pair; 0 never; -1 (the default) only batches of 1-4 queries, where
- * one launch of the two streaming contractions measured 4% faster on MI355X (above that it
- * was level or slower: each contraction alone fills the chip).  alto_set_fusion returns the
+ * on: 1 fuse every such pair; 0 never; -1 (the default) only batches of 1-4 queries on
+ * databases of at most 4 * tg * n records, where one launch of the two streaming
+ * contractions measured 4% faster on MI355X (elsewhere it was level or slower: each
+ * contraction alone fills the chip).  alto_set_fusion returns the
  * previous setting (-1, 0 or 1), -1 also on a null engine or a bad value;
  * alto_fused returns 1 if a batch of B queries runs fused with the current settings, 0 if
  * not, -1 on bad arguments. */

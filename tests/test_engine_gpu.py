@@ -391,7 +391,7 @@ def test_scan_and_packing_in_one_launch_give_the_same_bytes(hip_device, small_in
     # the built-in shape pairs run both contractions in one launch (B = 1 and 6 streaming,
     # 60 the 16x16 ring, 136 the 16x24 ring; 12 and 40 scan on 8- and 12-row tiles, which
     # have no fused pair): fused, unfused and oracle answers agree.  By default only batches
-    # of one A tile (1-4 queries) run fused.
+    # of one A tile (1-4 queries) run fused (on this database of 2048 <= 4 KP records).
     fusable = B not in (12, 40)
     inp = small_inputs
     reqs = workload.make_requests(B, inp.rows, inp.kp, seed=1200 + B)
