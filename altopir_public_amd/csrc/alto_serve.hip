@@ -42,15 +42,17 @@ int64_t now_ns() {
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 // Upload modes (alto_server_set_upload).
-constexpr int kCopyAtSubmit = 0, kZeroCopy = 1, kCopyAtDispatch = 2;
-// By default frames up to this size are copied per batch at dispatch: a copy call per frame
-// costs more than the transfer of a small frame, and kernels reading the pinned slots over
-// PCIe hold the CUs (1 GiB databases served at saturation: 58-68k QPS against 55-60k reading
-// the slots and 46.5k copying at submit).  Larger frames are copied at submit, so that their
-// transfer overlaps earlier batches' work (8 GiB: 22.7k against 16.6k at dispatch and 14.8k
-// reading the slots).
+constexpr int kCopyAtSubmit = 0, kZeroCopy = 1, kCopyAtDispatch = 2, kCopyBoth = 3;
+// By default frames up to this size are copied per batch at dispatch and the batch's
+// responses written to HBM and copied back per batch: a copy call per frame costs more than
+// the transfer of a small frame, and kernels reading or writing host memory over PCIe hold
+// the CUs (1 GiB databases served at saturation: 62-67k QPS with p99 11-14 ms, against
+// 54-68k with p99 10-22 ms copying only the frames, 55-60k reading the slots and 46.5k
+// copying at submit).  Larger frames are copied at submit, so that their transfer overlaps
+// earlier batches' work (8 GiB: 22.7k against 16-18k at dispatch and 14.8k reading the
+// slots).
 constexpr size_t kSmallFrame = 512 << 10;
-int default_upload(size_t frame_len) { return frame_len <= kSmallFrame ? kCopyAtDispatch : kCopyAtSubmit; }
+int default_upload(size_t frame_len) { return frame_len <= kSmallFrame ? kCopyBoth : kCopyAtSubmit; }
 
 struct Server;
 
@@ -65,6 +67,7 @@ struct Ctx {
     std::vector<int> batch;   // slots of the batch in flight
     std::vector<int> runs;    // the batch's slots, sorted (copy at dispatch)
     bool busy = false;
+    bool failed = false;      // a copy of the batch failed: every request reports -1
 };
 
 struct Db {
@@ -77,9 +80,11 @@ struct Db {
     // copied into d_req at submit; kZeroCopy the batch's kernels read the pinned slots over
     // PCIe; kCopyAtDispatch the batch's slots are copied into d_req on the batch's stream, a
     // few large copies per batch (slots go out in release order, so a batch's slots form a
-    // few contiguous runs)
+    // few contiguous runs); kCopyBoth that, and the responses go to d_resp_arena and are
+    // copied back the same way.  Otherwise the tail writes the responses over PCIe.
     std::atomic<int> upload_mode{0};
     uint8_t *h_resp = nullptr, *d_resp = nullptr;    // nslots x RL, mapped pinned
+    uint8_t *d_resp_arena = nullptr;                 // nslots x RL in HBM (kCopyBoth)
     std::vector<hipEvent_t> slot_ev;
     std::deque<int> free_slots;  // handed out from the front, returned at the back
     std::deque<int> pending;
@@ -136,6 +141,7 @@ void dispatcher(Server *s, int di) {
             d.pending.pop_front();
         }
         c.busy = true;
+        c.failed = false;
         const int B = (int)c.batch.size();
         lk.unlock();
         const int mode = d.upload_mode;
@@ -146,7 +152,7 @@ void dispatcher(Server *s, int di) {
             if (mode == kCopyAtSubmit) (void)hipStreamWaitEvent(c.stream, d.slot_ev[c.batch[i]], 0);
         }
         bool good = true;
-        if (mode == kCopyAtDispatch) {
+        if (mode == kCopyAtDispatch || mode == kCopyBoth) {
             // one copy per run of consecutive slots
             c.runs.assign(c.batch.begin(), c.batch.end());
             std::sort(c.runs.begin(), c.runs.end());
@@ -160,9 +166,20 @@ void dispatcher(Server *s, int di) {
             }
         }
         good = good && ok(hipMemcpyAsync(c.d_idx, c.h_idx, B * sizeof(int), hipMemcpyHostToDevice, c.stream), s, "index");
-        const int rc = good ? alto_answer_gather(c.ctx, zc ? d.h_req_dev : d.d_req, d.S, c.d_idx, B, d.d_resp, d.RL,
+        uint8_t *resp = mode == kCopyBoth ? d.d_resp_arena : d.d_resp;
+        const int rc = good ? alto_answer_gather(c.ctx, zc ? d.h_req_dev : d.d_req, d.S, c.d_idx, B, resp, d.RL,
                                                  c.d_idx, c.d_status, c.stream)
                             : -1;
+        // kCopyBoth: the responses were written to HBM; one copy back per run of slots
+        for (size_t i = 0; rc == 0 && mode == kCopyBoth && i < c.runs.size();) {
+            size_t j = i + 1;
+            while (j < c.runs.size() && c.runs[j] == c.runs[j - 1] + 1) j++;
+            const size_t off = (size_t)c.runs[i] * d.RL, bytes = (size_t)(c.runs[j - 1] - c.runs[i] + 1) * d.RL;
+            if (!ok(hipMemcpyAsync(d.h_resp + off, d.d_resp_arena + off, bytes, hipMemcpyDeviceToHost, c.stream), s,
+                    "response download"))
+                c.failed = true;  // applied at completion (the device still writes the statuses)
+            i = j;
+        }
         if (rc != 0) s->err = std::string("alto_answer_gather: ") + alto_last_error();
         (void)hipEventRecord(c.done, c.stream);
         lk.lock();
@@ -196,7 +213,7 @@ void completer(Server *s) {
             alto_completion x{};
             x.tag = d.tag[slot];
             x.db = di;
-            x.status = c.h_status[i];
+            x.status = c.failed ? -1 : c.h_status[i];
             x.slot = x.status == 0 ? slot : -1;
             x.response = x.status == 0 ? d.h_resp + (size_t)slot * d.RL : nullptr;
             x.response_len = x.status == 0 ? d.RL : 0;
@@ -235,7 +252,8 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
                ok(hipHostGetDevicePointer((void **)&d.h_req_dev, d.h_req, 0), s, "request mapping") &&
                ok(hipMalloc(&d.d_req, (size_t)d.nslots * d.S), s, "request arena") &&
                ok(hipHostMalloc((void **)&d.h_resp, (size_t)d.nslots * d.RL, hipHostMallocMapped), s, "responses") &&
-               ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping");
+               ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping") &&
+               ok(hipMalloc(&d.d_resp_arena, (size_t)d.nslots * d.RL), s, "response arena");
         good = good && ok(hipStreamCreateWithFlags(&d.upload, hipStreamNonBlocking), s, "upload stream");
         d.slot_ev.resize(d.nslots);
         for (auto &ev : d.slot_ev) good = good && ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), s, "event");
@@ -419,13 +437,14 @@ void alto_server_free(void *p) {
         if (d.h_req) (void)hipHostFree(d.h_req);
         if (d.d_req) (void)hipFree(d.d_req);
         if (d.h_resp) (void)hipHostFree(d.h_resp);
+        if (d.d_resp_arena) (void)hipFree(d.d_resp_arena);
     }
     delete s;
 }
 
 int alto_server_set_upload(void *p, int mode) {
     auto *s = (Server *)p;
-    if (!s || mode < -1 || mode > 2) return -1;
+    if (!s || mode < -1 || mode > 3) return -1;
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &dp : s->dbs) {
         if (!dp->pending.empty()) return -1;

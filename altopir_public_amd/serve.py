@@ -98,12 +98,13 @@ class Server:
     def set_upload(self, mode: int) -> None:
         """0: copy each frame into HBM at submit; 1: kernels read the pinned slots over PCIe;
         2: copy a batch's frames into HBM when it is dispatched (a few large copies);
-        -1: 2 for frames up to 512 KiB, else 0 (the default)."""
+        3: as 2, and write the responses to HBM and copy them back per batch;
+        -1: 3 for frames up to 512 KiB, else 0 (the default)."""
         if self.lib.alto_server_set_upload(self.ptr, mode) != 0:
             raise RuntimeError("alto_server_set_upload failed (requests pending or bad mode)")
 
     def upload_mode(self, db: int = 0) -> int:
-        """The upload mode in force for database db (0, 1 or 2)."""
+        """The upload mode in force for database db (0-3)."""
         return self.lib.alto_server_upload_mode(self.ptr, db)
 
     def contexts(self, db: int = 0) -> int:

@@ -268,10 +268,12 @@ int alto_server_set_graphs(void *srv, int mode);
  * the database's upload stream; 1 the batch's kernels read the frames straight from the
  * pinned slots over PCIe (no per-frame copy); 2 a batch's frames are copied into the device
  * arena when it is dispatched, one copy per run of consecutive slots, on the batch's stream;
- * -1 (the default) 2 for databases whose frames are at most 512 KiB, 0 above.  Only while
- * no request is pending or in flight; returns 0, -1 otherwise or on bad arguments. */
+ * 3 as 2, and the batch's responses are written to device memory and copied back the same
+ * way (in modes 0-2 the tail writes them to the pinned response slots over PCIe); -1 (the
+ * default) 3 for databases whose frames are at most 512 KiB, 0 above.  Only while no request
+ * is pending or in flight; returns 0, -1 otherwise or on bad arguments. */
 int alto_server_set_upload(void *srv, int mode);
-/* The upload mode in force for database db (0, 1 or 2 as above), -1 on bad arguments. */
+/* The upload mode in force for database db (0-3 as above), -1 on bad arguments. */
 int alto_server_upload_mode(void *srv, int db);
 
 /* Number of HBM bytes the engine holds for its resident state (database, packing

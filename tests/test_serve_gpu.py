@@ -30,16 +30,18 @@ def oracle_responses(inp, frames):
     return out, st
 
 
-@pytest.mark.parametrize("upload", [0, 1, 2, -1])
+@pytest.mark.parametrize("upload", [0, 1, 2, 3, -1])
 def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, upload):
     # upload 0: frames copied into HBM at submit; 1: the kernels read the pinned slots; 2: a
-    # batch's frames copied into HBM at dispatch; -1: the server's default
+    # batch's frames copied into HBM at dispatch; 3: that, and the responses copied back per
+    # batch; -1: the server's default
     inps = list(two_dbs)
     engines = [Engine.from_inputs(i) for i in inps]
     srv = Server(engines, max_batch=16, slots=64, contexts=2)
     srv.set_upload(upload)
-    # frames of these databases are small: the default copies them per batch at dispatch
-    assert [srv.upload_mode(d) for d in range(2)] == [upload if upload >= 0 else 2] * 2
+    # frames of these databases are small: by default a batch's frames and responses are
+    # copied at dispatch
+    assert [srv.upload_mode(d) for d in range(2)] == [upload if upload >= 0 else 3] * 2
     rng = np.random.default_rng(5)
     frames, dbs = [], []
     for k in range(40):
