@@ -46,7 +46,7 @@ constexpr int kCopyAtSubmit = 0, kZeroCopy = 1, kCopyAtDispatch = 2, kCopyBoth =
 // By default frames up to this size are copied per batch at dispatch and the batch's
 // responses written to HBM and copied back per batch: a copy call per frame costs more than
 // the transfer of a small frame, and kernels reading or writing host memory over PCIe hold
-// the CUs (1 GiB databases served at saturation: 62-67k QPS with p99 11-14 ms, against
+// the CUs (1 GiB databases served at saturation: 61-67k QPS with p99 11-14 ms, against
 // 54-68k with p99 10-22 ms copying only the frames, 55-60k reading the slots and 46.5k
 // copying at submit).  Larger frames are copied at submit, so that their transfer overlaps
 // earlier batches' work (8 GiB: 22.7k against 16-18k at dispatch and 14.8k reading the
