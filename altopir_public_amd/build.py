@@ -41,7 +41,9 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / "alto_engine.hip"), str(CSRC / "alto_serve.hip")]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", "-o", str(tmp), *srcs]
+    # the serving loop issues its uploads to the DMA engines through the HSA runtime
+    rocm_lib = Path(hipcc()).resolve().parent.parent / "lib"
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", "-o", str(tmp), *srcs, f"-L{rocm_lib}", "-lhsa-runtime64"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

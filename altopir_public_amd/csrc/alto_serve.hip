@@ -4,12 +4,18 @@
 // drains the whole queue into one synchronous alto_answer call per batch
 // (src/bin/altopir.rs:372-496).  Here every resident database keeps a pool of request
 // slots (pinned host staging + a device arena) and several engine contexts:
-//   submit   copies the frame into a free pinned slot and starts its upload at once on the
-//            database's upload stream (an event per slot marks its arrival in HBM);
+//   submit   copies the frame into a free pinned slot and hands it to the database's
+//            uploader thread (by default; other upload modes hand it straight to the
+//            dispatcher and copy a batch's frames when it is dispatched, or let the kernels
+//            read the pinned slots);
+//   upload   the uploader copies every frame handed to it since its last round into HBM,
+//            one DMA copy per run of consecutive slots, waits for them on the host and
+//            passes the frames on;
 //   dispatch whenever a context is free, takes every waiting request of that database (up
-//            to max_batch, never waiting to fill a batch), makes the context's stream wait
-//            for their uploads and answers them with alto_answer_gather; responses are
-//            written by the tail kernel straight into mapped pinned host memory;
+//            to max_batch, never waiting to fill a batch) and answers them with
+//            alto_answer_gather on the context's stream; responses are written by the tail
+//            kernel straight into mapped pinned host memory (or into HBM and copied back
+//            per batch);
 //   complete a completion thread waits for each batch, stamps the completion time and
 //            hands the tickets to poll().
 // Batches of one database overlap on the device as far as its contexts allow; the number
@@ -17,11 +23,14 @@
 // whose length is wrong get status 1 at once, and status 9 when every slot is taken
 // (src/bin/altopir.rs:417-436); the device validates the rest (statuses 2 and 3).
 #include <hip/hip_runtime.h>
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -41,18 +50,33 @@ int64_t now_ns() {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Writes every page of a fresh pinned allocation once, with a few threads.  Its pages are
+// mapped into the process on the first CPU access, one fault per page: without this, the
+// first frame written into each slot takes those faults inside submit (about 2,000 per
+// 8.45 MB frame), which slows the first pass over the slot pool several times over.
+void prefault(uint8_t *p, size_t bytes) {
+    constexpr size_t kChunk = 64 << 20;
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    const unsigned nt = (unsigned)std::min<size_t>(chunks, 8);
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nt; t++)
+        ts.emplace_back([&] {
+            for (size_t c; (c = next.fetch_add(1)) < chunks;)
+                memset(p + c * kChunk, 0, std::min(kChunk, bytes - c * kChunk));
+        });
+    for (auto &t : ts) t.join();
+}
+
 // Upload modes (alto_server_set_upload).
 constexpr int kCopyAtSubmit = 0, kZeroCopy = 1, kCopyAtDispatch = 2, kCopyBoth = 3;
-// By default frames up to this size are copied per batch at dispatch and the batch's
-// responses written to HBM and copied back per batch: a copy call per frame costs more than
-// the transfer of a small frame, and kernels reading or writing host memory over PCIe hold
-// the CUs (1 GiB databases served at saturation: 61-67k QPS with p99 11-14 ms, against
-// 54-68k with p99 10-22 ms copying only the frames, 55-60k reading the slots and 46.5k
-// copying at submit).  Larger frames are copied at submit, so that their transfer overlaps
-// earlier batches' work (8 GiB: 22.7k against 16-18k at dispatch and 14.8k reading the
-// slots).
-constexpr size_t kSmallFrame = 512 << 10;
-int default_upload(size_t frame_len) { return frame_len <= kSmallFrame ? kCopyBoth : kCopyAtSubmit; }
+// The default: frames copied by the uploader as they arrive, so that their transfer overlaps
+// the batches in flight, and responses written by the tail into the pinned slots.  Served at
+// saturation (batch 128, 2 contexts) this beats copying each batch's frames at dispatch with
+// or without copying its responses back: 92k against 81-88k QPS (1 GiB, p99 8.6 against
+// 8.7-14.5 ms), 24.4k against 23.7k (8 GiB), 3.6k against 3.5k (64 GiB); copying the
+// responses back as well (uploader frames) gave 79-87k, 22.5-23.4k and 3.55-3.6k.
+int default_upload(size_t) { return kCopyAtSubmit; }
 
 struct Server;
 
@@ -76,8 +100,8 @@ struct Db {
     int nslots = 0;
     uint8_t *h_req = nullptr, *d_req = nullptr;      // nslots x S (pinned, mapped / device arena)
     uint8_t *h_req_dev = nullptr;                    // device view of h_req
-    // how frames reach the device (alto_server_set_upload): kCopyAtSubmit each frame is
-    // copied into d_req at submit; kZeroCopy the batch's kernels read the pinned slots over
+    // how frames reach the device (alto_server_set_upload): kCopyAtSubmit the uploader
+    // thread copies frames into d_req as they arrive; kZeroCopy the batch's kernels read the pinned slots over
     // PCIe; kCopyAtDispatch the batch's slots are copied into d_req on the batch's stream, a
     // few large copies per batch (slots go out in release order, so a batch's slots form a
     // few contiguous runs); kCopyBoth that, and the responses go to d_resp_arena and are
@@ -85,15 +109,19 @@ struct Db {
     std::atomic<int> upload_mode{0};
     uint8_t *h_resp = nullptr, *d_resp = nullptr;    // nslots x RL, mapped pinned
     uint8_t *d_resp_arena = nullptr;                 // nslots x RL in HBM (kCopyBoth)
-    std::vector<hipEvent_t> slot_ev;
     std::deque<int> free_slots;  // handed out from the front, returned at the back
-    std::deque<int> pending;
+    std::deque<int> to_upload;   // submitted, not yet handed to the uploader
+    int uploading = 0;           // frames of the uploader's current round
+    std::condition_variable cv_upload;
+    std::deque<int> pending;     // in HBM (or readable by the kernels), waiting for a batch
     std::vector<uint64_t> tag;
     std::vector<int64_t> t_submit;
-    // one upload stream per database: with the contexts' streams it stays within the
-    // device's hardware queues, so an upload never queues behind a batch's kernels
-    hipStream_t upload = nullptr;
-    std::mutex up_mu;  // serialises uploads on this database's stream
+    // the uploader's copies go straight to the device's DMA engines and are waited for on
+    // this signal: nothing of them enters a stream, whose completion markers can sit in a
+    // hardware queue shared with a context's stream behind that batch's kernels (a waited-for
+    // upload stream returned only when the batch in flight ended)
+    hsa_signal_t up_sig{};
+    bool have_sig = false;
     std::vector<Ctx> ctx;
 };
 
@@ -109,12 +137,114 @@ struct Server {
     std::string err;
     alto_server_stats stats{};
     int device = 0;
+    bool hsa_up = false;               // hsa_init done (paired with hsa_shut_down)
+    hsa_agent_t gpu_agent{}, cpu_agent{};  // the device and a host agent, for the DMA copies
 };
+
+struct AgentSearch {
+    uint32_t bdf = 0, domain = 0;
+    hsa_agent_t gpu{}, cpu{};
+    bool gpu_found = false, cpu_found = false;
+};
+
+hsa_status_t match_agent(hsa_agent_t a, void *p) {
+    auto *f = (AgentSearch *)p;
+    hsa_device_type_t t;
+    if (hsa_agent_get_info(a, HSA_AGENT_INFO_DEVICE, &t) != HSA_STATUS_SUCCESS) return HSA_STATUS_SUCCESS;
+    if (t == HSA_DEVICE_TYPE_CPU && !f->cpu_found) {
+        f->cpu = a;
+        f->cpu_found = true;
+    } else if (t == HSA_DEVICE_TYPE_GPU) {
+        uint32_t bdf = 0, dom = 0;
+        if (hsa_agent_get_info(a, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_BDFID, &bdf) == HSA_STATUS_SUCCESS &&
+            hsa_agent_get_info(a, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_DOMAIN, &dom) == HSA_STATUS_SUCCESS &&
+            bdf == f->bdf && dom == f->domain) {
+            f->gpu = a;
+            f->gpu_found = true;
+        }
+    }
+    return HSA_STATUS_SUCCESS;
+}
+
+// The HSA agents of the server's device (matched by PCI location) and of the host, for the
+// uploader's DMA copies.
+bool find_agents(Server *s) {
+    if (hsa_init() != HSA_STATUS_SUCCESS) {
+        s->err = "hsa_init failed";
+        return false;
+    }
+    s->hsa_up = true;
+    char bus[64] = {0};
+    unsigned dom = 0, b = 0, dv = 0, fn = 0;
+    if (hipDeviceGetPCIBusId(bus, sizeof bus, s->device) != hipSuccess ||
+        sscanf(bus, "%x:%x:%x.%x", &dom, &b, &dv, &fn) != 4) {
+        s->err = "no PCI location for the device";
+        return false;
+    }
+    AgentSearch f;
+    f.bdf = (b << 8) | (dv << 3) | fn;
+    f.domain = dom;
+    (void)hsa_iterate_agents(match_agent, &f);
+    if (!f.gpu_found || !f.cpu_found) {
+        s->err = std::string("no HSA agent for device ") + bus;
+        return false;
+    }
+    s->gpu_agent = f.gpu;
+    s->cpu_agent = f.cpu;
+    return true;
+}
 
 bool ok(hipError_t e, Server *s, const char *what) {
     if (e == hipSuccess) return true;
     s->err = std::string(what) + ": " + hipGetErrorString(e);
     return false;
+}
+
+// Sorts `runs` (slots) and calls f(first slot, slot count) for each run of consecutive
+// slots; stops at the first call that returns false.
+template <class F>
+bool for_runs(std::vector<int> &runs, F f) {
+    std::sort(runs.begin(), runs.end());
+    for (size_t i = 0; i < runs.size();) {
+        size_t j = i + 1;
+        while (j < runs.size() && runs[j] == runs[j - 1] + 1) j++;
+        if (!f((size_t)runs[i], j - i)) return false;
+        i = j;
+    }
+    return true;
+}
+
+// One host-to-device copy per run of consecutive slots, on `st`.
+hipError_t copy_runs(const Db &d, std::vector<int> &runs, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    for_runs(runs, [&](size_t first, size_t n) {
+        e = hipMemcpyAsync(d.d_req + first * d.S, d.h_req + first * d.S, n * d.S, hipMemcpyHostToDevice, st);
+        return e == hipSuccess;
+    });
+    return e;
+}
+
+// One DMA copy per run of consecutive slots, all counted down on the database's signal;
+// returns once every issued copy has ended, true if all were issued and none failed.
+bool dma_runs(Server *s, Db &d, std::vector<int> &runs) {
+    size_t count = 0;
+    for_runs(runs, [&](size_t, size_t) { return ++count > 0; });
+    hsa_signal_store_screlease(d.up_sig, (hsa_signal_value_t)count);
+    size_t issued = 0;
+    for_runs(runs, [&](size_t first, size_t n) {
+        const hsa_status_t st = hsa_amd_memory_async_copy(d.d_req + first * d.S, s->gpu_agent, d.h_req + first * d.S,
+                                                          s->cpu_agent, n * d.S, 0, nullptr, d.up_sig);
+        if (st != HSA_STATUS_SUCCESS) return false;
+        issued++;
+        return true;
+    });
+    if (issued < count) hsa_signal_subtract_screlease(d.up_sig, (hsa_signal_value_t)(count - issued));
+    hsa_signal_value_t v;
+    // a failed copy leaves the signal negative
+    while ((v = hsa_signal_wait_scacquire(d.up_sig, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX,
+                                          HSA_WAIT_STATE_BLOCKED)) > 0) {
+    }
+    return issued == count && v == 0;
 }
 
 void dispatcher(Server *s, int di) {
@@ -149,29 +279,22 @@ void dispatcher(Server *s, int di) {
         for (int i = 0; i < B; i++) {
             c.h_idx[i] = c.batch[i];
             c.h_status[i] = 0;
-            if (mode == kCopyAtSubmit) (void)hipStreamWaitEvent(c.stream, d.slot_ev[c.batch[i]], 0);
         }
         bool good = true;
         if (mode == kCopyAtDispatch || mode == kCopyBoth) {
-            // one copy per run of consecutive slots
+            // one copy per run of consecutive slots (c.runs: the batch's slots, sorted)
             c.runs.assign(c.batch.begin(), c.batch.end());
-            std::sort(c.runs.begin(), c.runs.end());
-            for (size_t i = 0; good && i < c.runs.size();) {
-                size_t j = i + 1;
-                while (j < c.runs.size() && c.runs[j] == c.runs[j - 1] + 1) j++;
-                const size_t off = (size_t)c.runs[i] * d.S, bytes = (size_t)(c.runs[j - 1] - c.runs[i] + 1) * d.S;
-                good = ok(hipMemcpyAsync(d.d_req + off, d.h_req + off, bytes, hipMemcpyHostToDevice, c.stream), s,
-                          "batch upload");
-                i = j;
-            }
+            good = ok(copy_runs(d, c.runs, c.stream), s, "batch upload");
         }
         good = good && ok(hipMemcpyAsync(c.d_idx, c.h_idx, B * sizeof(int), hipMemcpyHostToDevice, c.stream), s, "index");
-        uint8_t *resp = mode == kCopyBoth ? d.d_resp_arena : d.d_resp;
+        const bool back = mode == kCopyBoth;
+        uint8_t *resp = back ? d.d_resp_arena : d.d_resp;
         const int rc = good ? alto_answer_gather(c.ctx, zc ? d.h_req_dev : d.d_req, d.S, c.d_idx, B, resp, d.RL,
                                                  c.d_idx, c.d_status, c.stream)
                             : -1;
         // kCopyBoth: the responses were written to HBM; one copy back per run of slots
-        for (size_t i = 0; rc == 0 && mode == kCopyBoth && i < c.runs.size();) {
+        // (c.runs: the batch's slots, sorted by the upload above)
+        for (size_t i = 0; rc == 0 && back && i < c.runs.size();) {
             size_t j = i + 1;
             while (j < c.runs.size() && c.runs[j] == c.runs[j - 1] + 1) j++;
             const size_t off = (size_t)c.runs[i] * d.RL, bytes = (size_t)(c.runs[j - 1] - c.runs[i] + 1) * d.RL;
@@ -190,6 +313,53 @@ void dispatcher(Server *s, int di) {
         s->stats.answered_in_batches += B;
         s->stats.max_batch_seen = std::max<long long>(s->stats.max_batch_seen, B);
         s->cv_done.notify_one();
+    }
+}
+
+// Frames copied on arrival: moves the frames handed over since the last round into HBM (at most
+// max_batch per round, so that a batch can start on the first ones while later ones are
+// still on their way) and then passes them, in submission order, to the dispatcher.
+void uploader(Server *s, int di) {
+    (void)hipSetDevice(s->device);
+    Db &d = *s->dbs[di];
+    std::vector<int> group, runs;
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        d.cv_upload.wait(lk, [&] { return s->stop || !d.to_upload.empty(); });
+        if (s->stop) return;
+        group.clear();
+        while (!d.to_upload.empty() && (int)group.size() < s->max_batch) {
+            group.push_back(d.to_upload.front());
+            d.to_upload.pop_front();
+        }
+        d.uploading = (int)group.size();
+        lk.unlock();
+        runs.assign(group.begin(), group.end());
+        const bool good = dma_runs(s, d, runs);
+        lk.lock();
+        d.uploading = 0;
+        if (good) {
+            for (int slot : group) d.pending.push_back(slot);
+            bool wake = false;
+            for (const Ctx &c : d.ctx) wake = wake || !c.busy;
+            if (wake) s->cv_dispatch.notify_all();
+        } else {
+            // these frames never reached the device: an engine failure for each
+            s->err = "request upload: a DMA copy failed";
+            const int64_t t = now_ns();
+            for (int slot : group) {
+                alto_completion x{};
+                x.tag = d.tag[slot];
+                x.db = di;
+                x.status = -1;
+                x.slot = -1;
+                x.t_submit_ns = d.t_submit[slot];
+                x.t_done_ns = t;
+                s->completed.push_back(x);
+                d.free_slots.push_back(slot);
+            }
+            s->cv_complete.notify_all();
+        }
     }
 }
 
@@ -239,7 +409,7 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
     s->max_batch = max_batch;
     (void)hipGetDevice(&s->device);
     for (int i = 0; i < n_engines; i++) s->dbs.emplace_back(new Db);
-    bool good = true;
+    bool good = find_agents(s);
     for (int i = 0; good && i < n_engines; i++) {
         Db &d = *s->dbs[i];
         d.engine = engines[i];
@@ -254,9 +424,15 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
                ok(hipHostMalloc((void **)&d.h_resp, (size_t)d.nslots * d.RL, hipHostMallocMapped), s, "responses") &&
                ok(hipHostGetDevicePointer((void **)&d.d_resp, d.h_resp, 0), s, "response mapping") &&
                ok(hipMalloc(&d.d_resp_arena, (size_t)d.nslots * d.RL), s, "response arena");
-        good = good && ok(hipStreamCreateWithFlags(&d.upload, hipStreamNonBlocking), s, "upload stream");
-        d.slot_ev.resize(d.nslots);
-        for (auto &ev : d.slot_ev) good = good && ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), s, "event");
+        if (good) {
+            prefault(d.h_req, (size_t)d.nslots * d.S);
+            prefault(d.h_resp, (size_t)d.nslots * d.RL);
+        }
+        if (good && hsa_signal_create(0, 0, nullptr, &d.up_sig) == HSA_STATUS_SUCCESS) d.have_sig = true;
+        else if (good) {
+            s->err = "upload signal";
+            good = false;
+        }
         d.tag.resize(d.nslots);
         d.t_submit.resize(d.nslots);
         for (int k = 0; k < d.nslots; k++) d.free_slots.push_back(k);
@@ -296,7 +472,10 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
         fprintf(stderr, "alto_server_new: %s\n", e.c_str());
         return nullptr;
     }
-    for (int i = 0; i < n_engines; i++) s->threads.emplace_back(dispatcher, s, i);
+    for (int i = 0; i < n_engines; i++) {
+        s->threads.emplace_back(dispatcher, s, i);
+        s->threads.emplace_back(uploader, s, i);
+    }
     s->threads.emplace_back(completer, s);
     return s;
 }
@@ -337,41 +516,23 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
             return reject;
         }
     }
-    uint8_t *h = d.h_req + (size_t)slot * d.S;
-    memcpy(h + ALTO_FRAME_OFFSET, frame, len);
-    hipError_t ue = hipSuccess;
-    if (d.upload_mode == kCopyAtSubmit) {
-        std::lock_guard<std::mutex> ul(d.up_mu);
-        ue = hipMemcpyAsync(d.d_req + (size_t)slot * d.S, h, ALTO_FRAME_OFFSET + len, hipMemcpyHostToDevice,
-                            d.upload);
-        if (ue == hipSuccess) ue = hipEventRecord(d.slot_ev[slot], d.upload);
-    }
-    if (ue != hipSuccess) {
-        // the frame never reached the device: report an engine failure for it at once
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->err = std::string("request upload: ") + hipGetErrorString(ue);
-        alto_completion x{};
-        x.tag = tag;
-        x.db = db;
-        x.status = -1;
-        x.slot = -1;
-        x.t_submit_ns = t0;
-        x.t_done_ns = now_ns();
-        s->completed.push_back(x);
-        d.free_slots.push_back(slot);
-        s->cv_complete.notify_all();
-        return -1;
-    }
-    bool wake = false;
+    memcpy(d.h_req + (size_t)slot * d.S + ALTO_FRAME_OFFSET, frame, len);
+    bool wake = false, upload = false;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         d.tag[slot] = tag;
         d.t_submit[slot] = t0;
-        d.pending.push_back(slot);
-        // the dispatcher waits for a free context; when all are busy, the completion of one
-        // wakes it and it then takes everything pending
-        for (const Ctx &c : d.ctx) wake = wake || !c.busy;
+        if (d.upload_mode == kCopyAtSubmit) {
+            d.to_upload.push_back(slot);
+            upload = true;
+        } else {
+            d.pending.push_back(slot);
+            // the dispatcher waits for a free context; when all are busy, the completion of
+            // one wakes it and it then takes everything pending
+            for (const Ctx &c : d.ctx) wake = wake || !c.busy;
+        }
     }
+    if (upload) d.cv_upload.notify_one();
     if (wake) s->cv_dispatch.notify_all();
     return 0;
 }
@@ -418,6 +579,7 @@ void alto_server_free(void *p) {
     s->cv_dispatch.notify_all();
     s->cv_done.notify_all();
     s->cv_complete.notify_all();
+    for (auto &dp : s->dbs) dp->cv_upload.notify_all();
     for (auto &t : s->threads) t.join();
     for (auto &dp : s->dbs) {
         Db &d = *dp;
@@ -430,15 +592,13 @@ void alto_server_free(void *p) {
             if (c.h_idx) (void)hipHostFree(c.h_idx);
             if (c.h_status) (void)hipHostFree(c.h_status);
         }
-        if (d.upload) (void)hipStreamSynchronize(d.upload);
-        for (auto &ev : d.slot_ev)
-            if (ev) (void)hipEventDestroy(ev);
-        if (d.upload) (void)hipStreamDestroy(d.upload);
+        if (d.have_sig) (void)hsa_signal_destroy(d.up_sig);
         if (d.h_req) (void)hipHostFree(d.h_req);
         if (d.d_req) (void)hipFree(d.d_req);
         if (d.h_resp) (void)hipHostFree(d.h_resp);
         if (d.d_resp_arena) (void)hipFree(d.d_resp_arena);
     }
+    if (s->hsa_up) (void)hsa_shut_down();
     delete s;
 }
 
@@ -447,10 +607,9 @@ int alto_server_set_upload(void *p, int mode) {
     if (!s || mode < -1 || mode > 3) return -1;
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &dp : s->dbs) {
-        if (!dp->pending.empty()) return -1;
+        if (!dp->pending.empty() || !dp->to_upload.empty() || dp->uploading) return -1;
         for (auto &c : dp->ctx)
             if (c.busy) return -1;
-        if (dp->upload) (void)hipStreamSynchronize(dp->upload);
     }
     for (auto &dp : s->dbs) dp->upload_mode = mode < 0 ? default_upload(dp->L) : mode;
     return 0;

@@ -96,10 +96,11 @@ class Server:
             raise RuntimeError("alto_server_set_graphs failed")
 
     def set_upload(self, mode: int) -> None:
-        """0: copy each frame into HBM at submit; 1: kernels read the pinned slots over PCIe;
+        """0: copy frames into HBM as they arrive (an upload thread per database);
+        1: kernels read the pinned slots over PCIe;
         2: copy a batch's frames into HBM when it is dispatched (a few large copies);
         3: as 2, and write the responses to HBM and copy them back per batch;
-        -1: 3 for frames up to 512 KiB, else 0 (the default)."""
+        -1: 0 (the default)."""
         if self.lib.alto_server_set_upload(self.ptr, mode) != 0:
             raise RuntimeError("alto_server_set_upload failed (requests pending or bad mode)")
 

@@ -126,9 +126,9 @@ def main() -> int:
     ap.add_argument("--serve-graphs", type=int, default=-1, choices=[-1, 0, 1],
                     help="graph mode of the served form's contexts (-1 follow the engine)")
     ap.add_argument("--serve-upload", type=int, default=-1, choices=[-1, 0, 1, 2, 3],
-                    help="served form: 0 copy frames into HBM at submit, 1 kernels read the pinned slots, "
+                    help="served form: 0 copy frames into HBM as they arrive, 1 kernels read the pinned slots, "
                          "2 copy a batch's frames into HBM at dispatch, 3 that and copy its responses back, "
-                         "-1 the server's default")
+                         "-1 the server's default (0)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -323,7 +323,7 @@ def main() -> int:
         served["form"] = (f"alto_server_* loop driven by alto_server_replay: {args.serve_threads} native submitter "
                           f"threads copy host frames into pinned slots, up to {win} requests outstanding")
         served["contexts"] = srv.contexts(0)
-        served["upload"] = {0: "frame copied into HBM at submit", 1: "kernels read the pinned slots",
+        served["upload"] = {0: "frames copied into HBM as they arrive", 1: "kernels read the pinned slots",
                             2: "batch copied into HBM at dispatch",
                             3: "batch copied into HBM at dispatch, responses copied back"}[srv.upload_mode(0)]
         srv.close()

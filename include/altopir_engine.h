@@ -264,14 +264,15 @@ int alto_server_replay(void *srv, int db, const uint8_t *const *frames, const si
 /* Graph mode of every context of the server (see alto_context_set_graphs; the server's
  * contexts start in -1, following their engine).  Returns 0, -1 on bad arguments. */
 int alto_server_set_graphs(void *srv, int mode);
-/* How frames reach the device: 0 each frame is copied into the device arena at submit, on
- * the database's upload stream; 1 the batch's kernels read the frames straight from the
- * pinned slots over PCIe (no per-frame copy); 2 a batch's frames are copied into the device
- * arena when it is dispatched, one copy per run of consecutive slots, on the batch's stream;
- * 3 as 2, and the batch's responses are written to device memory and copied back the same
- * way (in modes 0-2 the tail writes them to the pinned response slots over PCIe); -1 (the
- * default) 3 for databases whose frames are at most 512 KiB, 0 above.  Only while no request
- * is pending or in flight; returns 0, -1 otherwise or on bad arguments. */
+/* How frames reach the device: 0 (the default, also chosen by -1) frames are copied into the
+ * device arena as they arrive, by an upload thread of the database (one DMA copy per run of
+ * consecutive slots, waited for on the host, so that no upload waits behind a batch's
+ * kernels); 1 the batch's kernels read the frames straight from the pinned slots over PCIe
+ * (no per-frame copy); 2 a batch's frames are copied into the device arena when it is
+ * dispatched, one copy per run of consecutive slots, on the batch's stream; 3 as 2, and the
+ * batch's responses are written to device memory and copied back the same way (in modes 0-2
+ * the tail writes them to the pinned response slots over PCIe).  Only while no request is
+ * pending, uploading or in flight; returns 0, -1 otherwise or on bad arguments. */
 int alto_server_set_upload(void *srv, int mode);
 /* The upload mode in force for database db (0-3 as above), -1 on bad arguments. */
 int alto_server_upload_mode(void *srv, int db);

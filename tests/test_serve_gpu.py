@@ -32,16 +32,15 @@ def oracle_responses(inp, frames):
 
 @pytest.mark.parametrize("upload", [0, 1, 2, 3, -1])
 def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, upload):
-    # upload 0: frames copied into HBM at submit; 1: the kernels read the pinned slots; 2: a
+    # upload 0: frames copied into HBM as they arrive; 1: the kernels read the pinned slots; 2: a
     # batch's frames copied into HBM at dispatch; 3: that, and the responses copied back per
     # batch; -1: the server's default
     inps = list(two_dbs)
     engines = [Engine.from_inputs(i) for i in inps]
     srv = Server(engines, max_batch=16, slots=64, contexts=2)
     srv.set_upload(upload)
-    # frames of these databases are small: by default a batch's frames and responses are
-    # copied at dispatch
-    assert [srv.upload_mode(d) for d in range(2)] == [upload if upload >= 0 else 3] * 2
+    # by default frames are copied by the upload thread as they arrive
+    assert [srv.upload_mode(d) for d in range(2)] == [upload if upload >= 0 else 0] * 2
     rng = np.random.default_rng(5)
     frames, dbs = [], []
     for k in range(40):
@@ -95,6 +94,35 @@ def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, uploa
     srv.close()
     for e in engines:
         e.close()
+
+
+@pytest.mark.parametrize("upload", [0, 2, 3])
+def test_reused_slots_answer_each_new_frame(hip_device, two_dbs, upload):
+    # eight slots for 96 requests over 12 different frames: every slot is written again and
+    # again with another frame while the batches before it are answered, and each response
+    # must be the oracle's for the frame submitted last into that slot
+    inp = two_dbs[1]
+    eng = Engine.from_inputs(inp)
+    srv = Server([eng], max_batch=4, slots=8, contexts=2)
+    srv.set_upload(upload)
+    base = workload.make_requests(12, inp.rows, inp.kp, seed=4242)
+    want, st = oracle_responses(inp, base)
+    assert (st == 0).all()
+    total, sent, outstanding, got = 96, 0, 0, {}
+    while len(got) < total:
+        while sent < total and outstanding < 8:
+            assert srv.submit(0, bytes(base[sent % 12]), sent) == 0
+            sent += 1
+            outstanding += 1
+        for x in srv.poll(100):
+            got[x.tag] = x
+            outstanding -= 1
+    for i in range(total):
+        assert got[i].status == 0, i
+        assert got[i].response == want[i % 12].tobytes(), f"request {i} (frame {i % 12}) differs from the oracle"
+    assert max(x.batch for x in got.values()) > 1
+    srv.close()
+    eng.close()
 
 
 def test_poisson_load_reports_latency_and_rejects_corrupt_frames(hip_device, two_dbs):
