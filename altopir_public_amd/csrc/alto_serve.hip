@@ -116,12 +116,11 @@ struct Db {
     std::deque<int> pending;     // in HBM (or readable by the kernels), waiting for a batch
     std::vector<uint64_t> tag;
     std::vector<int64_t> t_submit;
-    // the uploader's copies go straight to the device's DMA engines and are waited for on
-    // this signal: nothing of them enters a stream, whose completion markers can sit in a
-    // hardware queue shared with a context's stream behind that batch's kernels (a waited-for
-    // upload stream returned only when the batch in flight ended)
-    hsa_signal_t up_sig{};
-    bool have_sig = false;
+    // the uploader's copies go straight to the device's DMA engines, one of these signals
+    // each, and are waited for on them: nothing of them enters a stream, whose completion
+    // markers can sit in a hardware queue shared with a context's stream behind that batch's
+    // kernels (a waited-for upload stream returned only when the batch in flight ended)
+    std::vector<hsa_signal_t> up_sigs;
     std::vector<Ctx> ctx;
 };
 
@@ -137,7 +136,6 @@ struct Server {
     std::string err;
     alto_server_stats stats{};
     int device = 0;
-    bool hsa_up = false;               // hsa_init done (paired with hsa_shut_down)
     hsa_agent_t gpu_agent{}, cpu_agent{};  // the device and a host agent, for the DMA copies
 };
 
@@ -169,11 +167,14 @@ hsa_status_t match_agent(hsa_agent_t a, void *p) {
 // The HSA agents of the server's device (matched by PCI location) and of the host, for the
 // uploader's DMA copies.
 bool find_agents(Server *s) {
-    if (hsa_init() != HSA_STATUS_SUCCESS) {
+    // one reference on the HSA runtime for the life of the process, as the HIP runtime holds
+    // its own: releasing it with hsa_shut_down when a server ends would look like the end of
+    // the process to tools that intercept that call (the profiler's tracing did)
+    static const hsa_status_t init = hsa_init();
+    if (init != HSA_STATUS_SUCCESS) {
         s->err = "hsa_init failed";
         return false;
     }
-    s->hsa_up = true;
     char bus[64] = {0};
     unsigned dom = 0, b = 0, dv = 0, fn = 0;
     if (hipDeviceGetPCIBusId(bus, sizeof bus, s->device) != hipSuccess ||
@@ -224,27 +225,39 @@ hipError_t copy_runs(const Db &d, std::vector<int> &runs, hipStream_t st) {
     return e;
 }
 
-// One DMA copy per run of consecutive slots, all counted down on the database's signal;
-// returns once every issued copy has ended, true if all were issued and none failed.
+// Signals per database for the uploader's copies: runs beyond this many in one round are
+// issued after the earlier ones have ended.
+constexpr size_t kUploadSignals = 16;
+
+// One DMA copy per run of consecutive slots, each completing its own signal (set to 1, counted
+// down by the copy); returns once every issued copy has ended, true if all were issued and
+// none failed.
 bool dma_runs(Server *s, Db &d, std::vector<int> &runs) {
-    size_t count = 0;
-    for_runs(runs, [&](size_t, size_t) { return ++count > 0; });
-    hsa_signal_store_screlease(d.up_sig, (hsa_signal_value_t)count);
-    size_t issued = 0;
+    bool good = true;
+    size_t k = 0;  // copies in flight: up_sigs[0, k)
+    auto drain = [&] {
+        for (size_t i = 0; i < k; i++) {
+            hsa_signal_value_t v;
+            while ((v = hsa_signal_wait_scacquire(d.up_sigs[i], HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX,
+                                                  HSA_WAIT_STATE_BLOCKED)) > 0) {
+            }
+            good = good && v == 0;  // a failed copy leaves its signal negative
+        }
+        k = 0;
+    };
     for_runs(runs, [&](size_t first, size_t n) {
-        const hsa_status_t st = hsa_amd_memory_async_copy(d.d_req + first * d.S, s->gpu_agent, d.h_req + first * d.S,
-                                                          s->cpu_agent, n * d.S, 0, nullptr, d.up_sig);
-        if (st != HSA_STATUS_SUCCESS) return false;
-        issued++;
+        if (k == d.up_sigs.size()) drain();
+        hsa_signal_store_screlease(d.up_sigs[k], 1);
+        if (hsa_amd_memory_async_copy(d.d_req + first * d.S, s->gpu_agent, d.h_req + first * d.S, s->cpu_agent,
+                                      n * d.S, 0, nullptr, d.up_sigs[k]) != HSA_STATUS_SUCCESS) {
+            good = false;
+            return false;
+        }
+        k++;
         return true;
     });
-    if (issued < count) hsa_signal_subtract_screlease(d.up_sig, (hsa_signal_value_t)(count - issued));
-    hsa_signal_value_t v;
-    // a failed copy leaves the signal negative
-    while ((v = hsa_signal_wait_scacquire(d.up_sig, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX,
-                                          HSA_WAIT_STATE_BLOCKED)) > 0) {
-    }
-    return issued == count && v == 0;
+    drain();
+    return good;
 }
 
 void dispatcher(Server *s, int di) {
@@ -428,10 +441,13 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
             prefault(d.h_req, (size_t)d.nslots * d.S);
             prefault(d.h_resp, (size_t)d.nslots * d.RL);
         }
-        if (good && hsa_signal_create(0, 0, nullptr, &d.up_sig) == HSA_STATUS_SUCCESS) d.have_sig = true;
-        else if (good) {
-            s->err = "upload signal";
-            good = false;
+        for (size_t k = 0; good && k < kUploadSignals; k++) {
+            hsa_signal_t sig;
+            if (hsa_signal_create(0, 0, nullptr, &sig) == HSA_STATUS_SUCCESS) d.up_sigs.push_back(sig);
+            else {
+                s->err = "upload signal";
+                good = false;
+            }
         }
         d.tag.resize(d.nslots);
         d.t_submit.resize(d.nslots);
@@ -592,13 +608,12 @@ void alto_server_free(void *p) {
             if (c.h_idx) (void)hipHostFree(c.h_idx);
             if (c.h_status) (void)hipHostFree(c.h_status);
         }
-        if (d.have_sig) (void)hsa_signal_destroy(d.up_sig);
+        for (hsa_signal_t sig : d.up_sigs) (void)hsa_signal_destroy(sig);
         if (d.h_req) (void)hipHostFree(d.h_req);
         if (d.d_req) (void)hipFree(d.d_req);
         if (d.h_resp) (void)hipHostFree(d.h_resp);
         if (d.d_resp_arena) (void)hipFree(d.d_resp_arena);
     }
-    if (s->hsa_up) (void)hsa_shut_down();
     delete s;
 }
 
