@@ -69,14 +69,14 @@ void prefault(uint8_t *p, size_t bytes) {
 }
 
 // Upload modes (alto_server_set_upload).
-constexpr int kCopyAtSubmit = 0, kZeroCopy = 1, kCopyAtDispatch = 2, kCopyBoth = 3;
+constexpr int kCopyOnArrival = 0, kZeroCopy = 1, kCopyAtDispatch = 2, kCopyBoth = 3;
 // The default: frames copied by the uploader as they arrive, so that their transfer overlaps
 // the batches in flight, and responses written by the tail into the pinned slots.  Served at
 // saturation (batch 128, 2 contexts) this beats copying each batch's frames at dispatch with
 // or without copying its responses back: 92k against 81-88k QPS (1 GiB, p99 8.6 against
 // 8.7-14.5 ms), 24.4k against 23.7k (8 GiB), 3.6k against 3.5k (64 GiB); copying the
 // responses back as well (uploader frames) gave 79-87k, 22.5-23.4k and 3.55-3.6k.
-int default_upload(size_t) { return kCopyAtSubmit; }
+int default_upload(size_t) { return kCopyOnArrival; }
 
 struct Server;
 
@@ -100,7 +100,7 @@ struct Db {
     int nslots = 0;
     uint8_t *h_req = nullptr, *d_req = nullptr;      // nslots x S (pinned, mapped / device arena)
     uint8_t *h_req_dev = nullptr;                    // device view of h_req
-    // how frames reach the device (alto_server_set_upload): kCopyAtSubmit the uploader
+    // how frames reach the device (alto_server_set_upload): kCopyOnArrival the uploader
     // thread copies frames into d_req as they arrive; kZeroCopy the batch's kernels read the pinned slots over
     // PCIe; kCopyAtDispatch the batch's slots are copied into d_req on the batch's stream, a
     // few large copies per batch (slots go out in release order, so a batch's slots form a
@@ -538,7 +538,7 @@ int alto_server_submit(void *p, int db, const uint8_t *frame, size_t len, uint64
         std::lock_guard<std::mutex> lk(s->mu);
         d.tag[slot] = tag;
         d.t_submit[slot] = t0;
-        if (d.upload_mode == kCopyAtSubmit) {
+        if (d.upload_mode == kCopyOnArrival) {
             d.to_upload.push_back(slot);
             upload = true;
         } else {
