@@ -118,7 +118,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2, help="request seed")
     ap.add_argument("--tune", action="store_true",
                     help="time every tile-shape variant per batch size first and keep the fastest")
-    ap.add_argument("--serve-batches", type=int, default=16,
+    ap.add_argument("--serve-batches", type=int, default=32,
                     help="batches' worth of requests pushed through the serving loop (0: skip)")
     ap.add_argument("--serve-threads", type=int, default=16, help="submitter threads of the served form")
     ap.add_argument("--serve-contexts", type=int, default=2, help="batches in flight in the served form")
