@@ -211,13 +211,17 @@ int alto_tune_get(const void *e, int which, size_t B);
 /* ------------------------------------------------------------- serving loop -- */
 
 /* A device-resident request loop over one or more engines (resident databases) with
- * continuous batching: submitted frames are uploaded at once into pinned/HBM slots, and
+ * continuous batching: a submitted frame is copied into a pinned slot and (by default) into
+ * HBM by the database's upload thread, with DMA copies through the HSA runtime (the library
+ * links libhsa-runtime64 and holds one reference on it for the life of the process), and
  * whenever one of a database's contexts is free, every waiting request of that database
  * (up to max_batch) is answered as one batch; batches of one database overlap on the
  * device as far as its contexts allow.  The number of contexts per database is at most
  * contexts_per_db and at most what fits in the device memory free at creation (each is
  * reserved for max_batch queries).  Responses are written by the device straight into
- * pinned host memory and stay valid until alto_server_release. */
+ * pinned host memory and stay valid until alto_server_release.  Creating a server writes
+ * every page of its pinned slot pools once (slots_per_db x frame size per database, about
+ * 1 s for 6.5 GB), so that no submit takes first-touch page faults. */
 typedef struct {
     uint64_t tag;          /* the caller's tag from alto_server_submit */
     int db;                /* database (engine index) */
@@ -238,8 +242,8 @@ typedef struct {
 void *alto_server_new(void *const *engines, int n_engines, int max_batch, int slots_per_db,
                       int contexts_per_db);
 /* Returns 0 (queued), 1 (wrong length) or 9 (no free slot; both completed at once with that
- * status), -1 on bad arguments or when the upload cannot be started (then completed at once
- * with status -1; alto_server_error names the cause). */
+ * status), -1 on bad arguments.  A frame whose upload fails later completes with status -1
+ * (alto_server_error names the cause). */
 int alto_server_submit(void *srv, int db, const uint8_t *frame, size_t len, uint64_t tag);
 /* Waits up to timeout_ms for completions; returns how many were written to out. */
 int alto_server_poll(void *srv, alto_completion *out, int max, int timeout_ms);
