@@ -220,8 +220,8 @@ int alto_tune_get(const void *e, int which, size_t B);
  * contexts_per_db and at most what fits in the device memory free at creation (each is
  * reserved for max_batch queries).  Responses are written by the device straight into
  * pinned host memory and stay valid until alto_server_release.  Creating a server writes
- * every page of its pinned slot pools once (slots_per_db x frame size per database, about
- * 1 s for 6.5 GB), so that no submit takes first-touch page faults. */
+ * every page of its pinned slot pools once (slots_per_db x frame size per database), so
+ * that no submit takes first-touch page faults. */
 typedef struct {
     uint64_t tag;          /* the caller's tag from alto_server_submit */
     int db;                /* database (engine index) */
