@@ -46,6 +46,8 @@ uint64_t host_mulmod(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __in
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
+
 // ------------------------------------------------------------- launch shapes --
 
 int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -945,9 +947,11 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
     e->ctx0 = new Context;
     bool ok = e->ctx0->init(e);
     hipStream_t setup = e->ctx0->stream;
+    // the attribute is process-wide per kernel, so it is set to the most any engine may ask
+    // for (an engine's own need would lower it under an earlier engine that needs more)
     ok = ok && HIP_OK(hipFuncSetAttribute((const void *)tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)std::min<size_t>(lds, 160 * 1024)));
-    if (e->tail_ok && lds > 160 * 1024) e->tail_ok = false;
+                                          (int)kMaxLds));
+    if (e->tail_ok && lds > kMaxLds) e->tail_ok = false;
     for (const auto &v : kVariants) {
         v.prepare_scan();
         v.prepare_pack();

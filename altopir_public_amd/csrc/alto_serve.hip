@@ -16,8 +16,9 @@
 //            alto_answer_gather on the context's stream; responses are written by the tail
 //            kernel straight into mapped pinned host memory (or into HBM and copied back
 //            per batch);
-//   complete a completion thread waits for each batch, stamps the completion time and
-//            hands the tickets to poll().
+//   complete a completion thread per context waits for that context's batch, stamps the
+//            completion time and hands the tickets to poll(), so batches complete in the
+//            order they end, whatever database they belong to.
 // Batches of one database overlap on the device as far as its contexts allow; the number
 // of contexts is bounded by the device memory free at start (memory admission).  Frames
 // whose length is wrong get status 1 at once, and status 9 when every slot is taken
@@ -80,6 +81,8 @@ int default_upload(size_t) { return kCopyOnArrival; }
 
 struct Server;
 
+constexpr int kNoGraphChange = 2;  // Ctx::graph_req: no graph-mode change pending
+
 struct Ctx {
     void *ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -91,7 +94,11 @@ struct Ctx {
     std::vector<int> batch;   // slots of the batch in flight
     std::vector<int> runs;    // the batch's slots, sorted (copy at dispatch)
     bool busy = false;
+    bool launched = false;    // the batch is on the device; the context's completer waits for it
     bool failed = false;      // a copy of the batch failed: every request reports -1
+    // graph mode asked for by alto_server_set_graphs, applied by the dispatcher before the
+    // context's next batch (only that thread drives the context): kNoGraphChange = none
+    int graph_req = kNoGraphChange;
 };
 
 struct Db {
@@ -110,6 +117,7 @@ struct Db {
     uint8_t *h_resp = nullptr, *d_resp = nullptr;    // nslots x RL, mapped pinned
     uint8_t *d_resp_arena = nullptr;                 // nslots x RL in HBM (kCopyBoth)
     std::deque<int> free_slots;  // handed out from the front, returned at the back
+    std::vector<uint8_t> leased; // per slot: its response is handed out and not yet released
     std::deque<int> to_upload;   // submitted, not yet handed to the uploader
     int uploading = 0;           // frames of the uploader's current round
     std::condition_variable cv_upload;
@@ -130,7 +138,6 @@ struct Server {
     std::mutex mu;
     std::condition_variable cv_dispatch, cv_done, cv_complete;
     std::deque<alto_completion> completed;
-    std::deque<std::pair<int, int>> inflight;  // (db, ctx) in launch order
     std::atomic<bool> stop{false};
     std::vector<std::thread> threads;
     std::string err;
@@ -286,7 +293,10 @@ void dispatcher(Server *s, int di) {
         c.busy = true;
         c.failed = false;
         const int B = (int)c.batch.size();
+        const int graph_req = c.graph_req;
+        c.graph_req = kNoGraphChange;
         lk.unlock();
+        if (graph_req != kNoGraphChange) (void)alto_context_set_graphs(c.ctx, graph_req);
         const int mode = d.upload_mode;
         const bool zc = mode == kZeroCopy;
         for (int i = 0; i < B; i++) {
@@ -321,11 +331,11 @@ void dispatcher(Server *s, int di) {
         lk.lock();
         if (rc != 0)
             for (int i = 0; i < B; i++) c.h_status[i] = -1;  // reported as an engine failure
-        s->inflight.emplace_back(di, ci);
+        c.launched = true;
         s->stats.batches++;
         s->stats.answered_in_batches += B;
         s->stats.max_batch_seen = std::max<long long>(s->stats.max_batch_seen, B);
-        s->cv_done.notify_one();
+        s->cv_done.notify_all();
     }
 }
 
@@ -376,16 +386,17 @@ void uploader(Server *s, int di) {
     }
 }
 
-void completer(Server *s) {
+// One completer per context: it waits for that context's batch alone, so a batch of a
+// small database is completed as soon as it ends, not behind an earlier batch of a larger
+// one on another context.
+void completer(Server *s, int di, int ci) {
     (void)hipSetDevice(s->device);
+    Db &d = *s->dbs[di];
+    Ctx &c = d.ctx[ci];
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        s->cv_done.wait(lk, [&] { return s->stop || !s->inflight.empty(); });
-        if (s->inflight.empty() && s->stop) return;
-        auto [di, ci] = s->inflight.front();
-        s->inflight.pop_front();
-        Db &d = *s->dbs[di];
-        Ctx &c = d.ctx[ci];
+        s->cv_done.wait(lk, [&] { return s->stop || c.launched; });
+        if (!c.launched && s->stop) return;
         lk.unlock();
         (void)hipEventSynchronize(c.done);
         const int64_t t = now_ns();
@@ -404,8 +415,10 @@ void completer(Server *s) {
             x.t_done_ns = t;
             x.batch = B;
             if (x.status != 0) d.free_slots.push_back(slot);  // nothing to read back
+            else d.leased[slot] = 1;                          // until alto_server_release
             s->completed.push_back(x);
         }
+        c.launched = false;
         c.busy = false;
         s->cv_complete.notify_all();
         s->cv_dispatch.notify_all();
@@ -450,6 +463,7 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
             }
         }
         d.tag.resize(d.nslots);
+        d.leased.assign(d.nslots, 0);
         d.t_submit.resize(d.nslots);
         for (int k = 0; k < d.nslots; k++) d.free_slots.push_back(k);
         // memory admission: as many contexts (each reserved for max_batch queries) as fit in
@@ -491,8 +505,8 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
     for (int i = 0; i < n_engines; i++) {
         s->threads.emplace_back(dispatcher, s, i);
         s->threads.emplace_back(uploader, s, i);
+        for (int k = 0; k < (int)s->dbs[i]->ctx.size(); k++) s->threads.emplace_back(completer, s, i, k);
     }
-    s->threads.emplace_back(completer, s);
     return s;
 }
 
@@ -571,7 +585,10 @@ int alto_server_release(void *p, int db, int slot) {
     auto *s = (Server *)p;
     if (!s || db < 0 || db >= (int)s->dbs.size() || slot < 0 || slot >= s->dbs[db]->nslots) return -1;
     std::lock_guard<std::mutex> lk(s->mu);
-    s->dbs[db]->free_slots.push_back(slot);  // no dispatcher waits on slots
+    Db &d = *s->dbs[db];
+    if (!d.leased[slot]) return -1;  // not handed out with a response, or released already
+    d.leased[slot] = 0;
+    d.free_slots.push_back(slot);  // no dispatcher waits on slots
     return 0;
 }
 
@@ -639,12 +656,11 @@ int alto_server_upload_mode(void *p, int db) {
 int alto_server_set_graphs(void *p, int mode) {
     auto *s = (Server *)p;
     if (!s || mode < -1 || mode > 1) return -1;
+    // recorded here and applied by each context's dispatcher before its next batch: a busy
+    // context may be inside alto_answer_gather (capturing or launching a graph) right now
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &dp : s->dbs)
-        for (auto &c : dp->ctx) {
-            if (c.busy) (void)hipStreamSynchronize(c.stream);
-            if (alto_context_set_graphs(c.ctx, mode) != 0) return -1;
-        }
+        for (auto &c : dp->ctx) c.graph_req = mode;
     return 0;
 }
 

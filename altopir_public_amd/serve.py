@@ -76,6 +76,7 @@ class Done:
     t_submit_ns: int
     t_done_ns: int
     batch: int
+    slot: int = -1
 
 
 class Server:
@@ -115,9 +116,10 @@ class Server:
         a = np.frombuffer(frame, np.uint8) if isinstance(frame, (bytes, bytearray)) else np.ascontiguousarray(frame)
         return self.lib.alto_server_submit(self.ptr, db, a.ctypes.data if len(a) else None, len(a), tag)
 
-    def poll(self, timeout_ms: int = 10, copy: bool = True) -> list[Done]:
+    def poll(self, timeout_ms: int = 10, copy: bool = True, release: bool = True) -> list[Done]:
         """Completions so far (waits up to timeout_ms for the first).  A valid request's
-        response is copied out (copy=True) and its slot released either way."""
+        response is copied out (copy=True) and its slot released (release=True; otherwise
+        the caller releases Done.slot with release())."""
         k = self.lib.alto_server_poll(self.ptr, self._buf, len(self._buf), timeout_ms)
         out = []
         for i in range(max(k, 0)):
@@ -126,9 +128,15 @@ class Server:
             if c.status == 0:
                 if copy:
                     resp = C.string_at(c.response, c.response_len)
-                self.lib.alto_server_release(self.ptr, c.db, c.slot)
-            out.append(Done(c.tag, c.db, c.status, resp, c.t_submit_ns, c.t_done_ns, c.batch))
+                if release:
+                    self.lib.alto_server_release(self.ptr, c.db, c.slot)
+            out.append(Done(c.tag, c.db, c.status, resp, c.t_submit_ns, c.t_done_ns, c.batch, c.slot))
         return out
+
+    def release(self, db: int, slot: int) -> int:
+        """Returns a leased response slot (alto_server_release): 0, or -1 for a slot that is
+        not leased (never handed out with a response, or released already)."""
+        return self.lib.alto_server_release(self.ptr, db, slot)
 
     def replay(self, frames, db: int = 0, total: int = 1024, threads: int = 8, window: int = 384) -> dict:
         """Native load driver (alto_server_replay): `threads` submitter threads keep up to

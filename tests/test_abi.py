@@ -150,3 +150,42 @@ def test_variant_table_and_tuning_buckets():
     for b, top in enumerate(tops):
         assert autotune.bucket(4 * top) == b and autotune.bucket(4 * top + 1) == min(b + 1, 29)
     assert autotune.bucket(1) == 0 and autotune.bucket(10**7) == 29
+
+
+def test_cpp_mirror_compiles_and_drives_the_host_entry_points(tmp_path):
+    # include/altopir/engine.hpp built into a program linked against the library; its tiled
+    # database and packing matrix are compared with the oracle's, and its checks and error
+    # messages follow src/engine.rs
+    import subprocess
+
+    root = Path(__file__).resolve().parent.parent
+    exe = tmp_path / "engine_hpp_check"
+    lib = _lib.LIB_PATH
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-I{root / 'include'}",
+                    str(root / "tests" / "cpp" / "engine_hpp_check.cpp"), "-o", str(exe), f"-L{lib.parent}",
+                    "-laltopir_hip", f"-Wl,-rpath,{lib.parent}", "-Wl,-rpath-link,/opt/rocm/lib"], check=True)
+    r = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert r.stdout.split("\n")[:-1] == ["ok parse_method", "ok init_amx", "ok MappedBytes", "ok TiledDb",
+                                          "ok build_packing_matrix", "ok Engine checks"]
+    orc = oc.oracle()
+    rows, cols, ld = 256, 64, 320
+    db = np.fromfile(tmp_path / "db.bin", np.uint8)
+    want = np.zeros(rows * cols, np.uint8)
+    orc.orc_tile_db(want.ctypes.data, db.ctypes.data, ld, rows, cols)
+    assert np.array_equal(np.fromfile(tmp_path / "tiled.bin", np.uint8), want)
+    n, tg, rho = 16, 8, 2
+    num_iter = n // 2 - 1
+    br = [workload.bitrev(j, 4) for j in range(n)]
+    tables = np.array([[br[((t * (2 * br[j] + 1)) % (2 * n) - 1) // 2] for j in range(n)]
+                       for t in range(1, 2 * n, 2)], dtype=np.uint32).reshape(-1)
+    gen_pows = np.array([pow(5, i, 2 * n) for i in range(n)], dtype=np.uint32)
+    bt = [np.fromfile(tmp_path / f"bt{o}.bin", np.uint64) for o in range(rho)]
+    bb = [np.fromfile(tmp_path / f"bb{o}.bin", np.uint64) for o in range(rho)]
+    assert np.array_equal(bt[0], synthetic.stream(70, num_iter * tg * n))
+    pt = (C.c_void_p * rho)(*[x.ctypes.data for x in bt])
+    pb = (C.c_void_p * rho)(*[x.ctypes.data for x in bb])
+    ub = np.zeros(4 * tg * n * rho * n, np.int8)
+    orc.orc_pack_build(ub.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
+                       num_iter, tg, n, rho)
+    assert np.array_equal(np.fromfile(tmp_path / "ub.bin", np.int8), ub)

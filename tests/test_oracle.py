@@ -302,6 +302,25 @@ def test_rejections_match_the_compiled_reference():
 
 
 @needs_ref
+@pytest.mark.parametrize("rows,cols,ld", [(64, 32, 64), (256, 96, 320), (1024, 64, 1024), (192, 160, 200)])
+def test_tile_db_matches_the_compiled_reference(rows, cols, ld):
+    # orc_tile_db and the library's host sw_tile_db against the reference's own sw_tile_db
+    # (src/amx_kernels.cpp:138-149), including column pitches above the record count
+    from altopir_public_amd import _lib
+
+    rlib, orc = oc.ref(), oc.oracle()
+    db = synthetic.uniform_bytes(rows + cols + ld, cols * ld)
+    want = np.zeros(rows * cols, np.uint8)
+    rlib.sw_tile_db(want.ctypes.data, db.ctypes.data, ld, rows, cols, 3)
+    got_o = np.full(rows * cols, 0x5A, np.uint8)
+    orc.orc_tile_db(got_o.ctypes.data, db.ctypes.data, ld, rows, cols)
+    assert np.array_equal(got_o, want)
+    got_l = np.full(rows * cols, 0x5A, np.uint8)
+    _lib.load().sw_tile_db(got_l.ctypes.data, db.ctypes.data, ld, rows, cols, 2)
+    assert np.array_equal(got_l, want)
+
+
+@needs_ref
 def test_pack_build_matches_the_compiled_reference():
     lib, rlib = oc.oracle(), oc.ref()
     n, tg, rho = 32, 4, 2

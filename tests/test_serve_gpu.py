@@ -96,7 +96,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, uploa
         e.close()
 
 
-@pytest.mark.parametrize("upload", [0, 2, 3])
+@pytest.mark.parametrize("upload", [0, 1, 2, 3])
 def test_reused_slots_answer_each_new_frame(hip_device, two_dbs, upload):
     # eight slots for 96 requests over 12 different frames: every slot is written again and
     # again with another frame while the batches before it are answered, and each response
@@ -185,3 +185,112 @@ def test_load_generator_end_to_end(hip_device, capsys):
     assert r["rejected_as_expected"] and r["sent"] == 300
     assert r["answered"] + sum(r["rejected"].values()) == 300
     assert r["p99_ms"] >= r["p50_ms"] > 0
+
+
+def test_release_frees_only_a_leased_slot(hip_device, two_dbs):
+    # a slot goes back to the pool once: a second release, or the release of a slot that was
+    # never handed out with a response, is refused (two requests would otherwise share a slot)
+    inp = two_dbs[0]
+    eng = Engine.from_inputs(inp)
+    srv = Server([eng], max_batch=4, slots=4, contexts=1)
+    assert srv.release(0, 0) == -1
+    frames = workload.make_requests(4, inp.rows, inp.kp, seed=31)
+    want, _ = oracle_responses(inp, frames)
+    assert srv.submit(0, bytes(frames[0]), 0) == 0
+    got = []
+    while not got:
+        got = srv.poll(100, release=False)
+    slot = got[0].slot
+    assert got[0].status == 0 and 0 <= slot < 4
+    assert srv.release(0, slot) == 0
+    assert srv.release(0, slot) == -1
+    # four slots again: four requests in flight at once, each answered with its own response
+    for i in range(4):
+        assert srv.submit(0, bytes(frames[i]), 10 + i) == 0
+    done = {}
+    while len(done) < 4:
+        for x in srv.poll(100, release=False):
+            done[x.tag] = x
+    assert sorted(x.slot for x in done.values()) == [0, 1, 2, 3]
+    for i in range(4):
+        assert done[10 + i].response == want[i].tobytes()
+        assert srv.release(0, done[10 + i].slot) == 0
+    srv.close()
+    eng.close()
+
+
+def test_set_graphs_applies_between_batches(hip_device, two_dbs):
+    # the graph mode changes while batches are in flight; each context takes it before its
+    # next batch, and every response stays the oracle's
+    inp = two_dbs[1]
+    eng = Engine.from_inputs(inp)
+    srv = Server([eng], max_batch=4, slots=32, contexts=2)
+    frames = workload.make_requests(6, inp.rows, inp.kp, seed=4343)
+    want, _ = oracle_responses(inp, frames)
+    got = {}
+    for k, mode in enumerate((1, 0, 1, -1)):
+        for i in range(12):
+            assert srv.submit(0, bytes(frames[i % 6]), 100 * k + i) == 0
+        assert srv.set_graphs(mode) is None
+        while sum(1 for t in got if t // 100 == k) < 12:
+            for x in srv.poll(100):
+                got[x.tag] = x
+    for t, x in got.items():
+        assert x.status == 0 and x.response == want[(t % 100) % 6].tobytes(), t
+    srv.close()
+    eng.close()
+
+
+def test_small_database_completes_ahead_of_a_long_batch(hip_device):
+    # a long batch on an 8 GiB database is on the device when one request for a small
+    # database arrives: each context has its own completer, so the small request completes
+    # when its own batch ends, not behind the long one
+    import torch
+
+    from altopir_public_amd import synthetic
+
+    rows, cols = 262144, 32768
+    big = workload.make_inputs(rows, cols, n=2048, tg=4, seed=0, db=False)
+    d_db = synthetic.uniform_bytes_device(0, rows * cols)
+    torch.cuda.synchronize()
+    e_big = Engine.from_device_db(d_db.data_ptr(), big)
+    del d_db
+    torch.cuda.empty_cache()
+    small = workload.make_inputs(2048, 4096, n=2048, tg=4, seed=13)
+    e_small = Engine.from_inputs(small)
+    L = big.request_len
+    stride = (7 + L + 255) // 256 * 256
+    arena = workload.make_requests_device(8, rows, big.kp, seed=3, stride=stride).cpu().numpy()
+    bframes = [np.ascontiguousarray(arena[c, 7:7 + L]) for c in range(8)]
+    sframe = workload.make_requests(1, small.rows, small.kp, seed=61)[0]
+    want_small, _ = oracle_responses(small, [sframe])
+    srv = Server([e_big, e_small], max_batch=256, slots=320, contexts=1)
+    srv.set_upload(2)  # frames copied at dispatch, on the batch's stream
+    # 255 requests from 8 threads: while the first batches run, the rest pile up and go out
+    # in a batch of dozens (several ms of device work)
+    ts = [threading.Thread(target=lambda k=k: [srv.submit(0, bframes[i % 8], i) for i in range(k, 255, 8)])
+          for k in range(8)]
+    for t in ts:
+        t.start()
+    done = {}
+    while srv.stats()["max_batch_seen"] < 32:
+        for x in srv.poll(1):
+            done[x.tag] = x
+        assert len(done) < 255, "no batch of 32 or more requests formed"
+    mb = srv.stats()["max_batch_seen"]  # launched before the small request is submitted
+    assert srv.submit(1, bytes(sframe), 1000) == 0
+    for t in ts:
+        t.join()
+    while len(done) < 256:
+        for x in srv.poll(100):
+            done[x.tag] = x
+    t_long = min(x.t_done_ns for x in done.values() if x.db == 0 and x.batch == mb)
+    assert all(x.status == 0 for x in done.values())
+    s = done[1000]
+    assert s.response == want_small[0].tobytes()
+    assert s.t_done_ns < t_long, (
+        f"the small database's request completed {(s.t_done_ns - t_long) / 1e6:.2f} ms after the batch of {mb} "
+        f"that was on the device when it arrived")
+    srv.close()
+    e_big.close()
+    e_small.close()
