@@ -118,7 +118,8 @@ def run_batch(db, B, seed):
         slots[1, w:w + 4] = torch.tensor(list(int(synthetic.Q).to_bytes(4, "little")), dtype=torch.uint8,
                                          device=slots.device)
         want[1] = 3
-    status = torch.full((B,), -1, dtype=torch.int32, device="cuda")
+    # statuses start at 0: the device entry point keeps a nonzero status given on entry
+    status = torch.zeros(B, dtype=torch.int32, device="cuda")
     out = torch.empty((B, rl), dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
     log(f"{rows} records, batch {B}: alto_answer_device")
