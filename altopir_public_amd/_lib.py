@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libaltopir_hip.so"
 SYMBOLS = (
     "sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build",
     "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
-    "alto_answer_device", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
+    "alto_answer_device", "alto_answer_device_noalloc", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
     "alto_engine_new_device", "alto_engine_new_records", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
     "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
@@ -64,6 +64,8 @@ def load() -> C.CDLL:
     lib.alto_answer.argtypes = [P, P, P, SZ, P, P, C.c_int, P]
     lib.alto_answer_device.restype = C.c_int
     lib.alto_answer_device.argtypes = [P, P, SZ, SZ, P, SZ, P, P]
+    lib.alto_answer_device_noalloc.restype = C.c_int
+    lib.alto_answer_device_noalloc.argtypes = [P, P, SZ, SZ, P, SZ, P, P]
     lib.alto_engine_device_bytes.restype = SZ
     lib.alto_engine_device_bytes.argtypes = [P]
     lib.alto_last_error.restype = C.c_char_p

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -399,6 +400,21 @@ struct Engine {
 
 // ------------------------------------------------------------------ contexts --
 
+// Per-batch device workspace of one batch of up to `cap` queries (cap a multiple of 4).
+struct Workspace {
+    int8_t *qa;                // scan limbs, cap * 4 * rows bytes
+    int8_t *ya;                // packing limbs, cap * 4 * KP bytes
+    uint32_t *acc;             // scan accumulators, cap * C
+    uint32_t *beta;            // packing results, cap * C
+    uint32_t *qsum;            // cap
+    unsigned long long *ysum;  // cap
+};
+
+// Bytes of the workspace of a batch of Bp queries (Bp a multiple of 4), each buffer starting
+// on a 256-byte boundary, and its layout inside one block of memory at `base`.
+size_t workspace_bytes(const Engine &e, size_t Bp);
+Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base);
+
 // One batch in flight: its workspace, stream, timing events and captured graphs.  Batches
 // on different contexts of one engine may run concurrently; one context runs one batch at
 // a time.
@@ -555,8 +571,9 @@ struct Context {
         return ok;
     }
 
-    int enqueue(FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx, int *status, hipStream_t st,
-                bool timed, bool record_start = true);
+    Workspace ws() const { return Workspace{qa, ya, acc, beta, qsum, ysum}; }
+    int enqueue(const Workspace &w, FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx,
+                int *status, hipStream_t st, bool timed, bool record_start = true);
     int answer_device(FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx, int *status,
                       hipStream_t st);
 };
@@ -568,14 +585,41 @@ Engine::~Engine() {
         if (p) (void)hipFree(p);
 }
 
-int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx, int *st_dev,
-                     hipStream_t st, bool timed, bool record_start) {
-    const Engine &E = *e;
+size_t workspace_bytes(const Engine &e, size_t Bp) {
+    return round_up(Bp * 4 * e.rows, 256) + round_up(Bp * 4 * e.kp, 256) + 2 * round_up(Bp * e.cols * 4, 256) +
+           round_up(Bp * 4, 256) + round_up(Bp * 8, 256);
+}
+
+Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base) {
+    Workspace w;
+    uint8_t *p = base;
+    auto take = [&](size_t bytes) {
+        uint8_t *q = p;
+        p += round_up(bytes, 256);
+        return q;
+    };
+    w.qa = (int8_t *)take(Bp * 4 * e.rows);
+    w.ya = (int8_t *)take(Bp * 4 * e.kp);
+    w.acc = (uint32_t *)take(Bp * e.cols * 4);
+    w.beta = (uint32_t *)take(Bp * e.cols * 4);
+    w.qsum = (uint32_t *)take(Bp * 4);
+    w.ysum = (unsigned long long *)take(Bp * 8);
+    return w;
+}
+
+// The launch chain of one batch on stream st with workspace w: validate, limb preparation,
+// the two contractions (or both in one launch), tail.  ev: the stage events to record at the
+// launch boundaries (null: none); *fused is set when both contractions ran in one launch.
+int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, size_t out_stride,
+                 const int *oidx, int *st_dev, hipStream_t st, hipEvent_t *ev, bool record_start, bool *fused) {
+    int8_t *const qa = w.qa, *const ya = w.ya;
+    uint32_t *const acc = w.acc, *const beta = w.beta, *const qsum = w.qsum;
+    unsigned long long *const ysum = w.ysum;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
-    if (timed && record_start) record(ev[0], st);
+    if (ev && record_start) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(E.kp)), dim3(256), 0, st, fs, st_dev,
                        (uint32_t)E.kp, qsum, ysum);
-    if (timed) record(ev[1], st);
+    if (ev) (void)hipEventRecord(ev[1], st);
     GemmArgs gs{};  // scan: query limbs x database -> acc
     gs.KC = (int)E.KC;
     gs.MT = (int)MT;
@@ -613,20 +657,20 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     pa.nfill = (unsigned)std::min<size_t>(1024, (pa.acc4 + 1023) / 1024);
     const unsigned prep_blocks = pa.sx * pa.sy + pa.px * pa.py + pa.nfill;
     hipLaunchKernelGGL(prep_kernel, dim3(prep_blocks), dim3(256), 0, st, pa);
-    if (timed) record(ev[2], st);
+    if (ev) (void)hipEventRecord(ev[2], st);
     const int fz = E.fused_for((int)MT);
     if (fz >= 0) {
         // one launch for both contractions; the stage times show the pair under "scan"
         // (packing preparation and packing read as zero)
         kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
-        if (timed) record(ev[3], st);
+        if (ev) (void)hipEventRecord(ev[3], st);
     } else {
         kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
-        if (timed) record(ev[3], st);
+        if (ev) (void)hipEventRecord(ev[3], st);
         kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
-        if (timed) record(ev[5], st);
+        if (ev) (void)hipEventRecord(ev[5], st);
     }
-    if (timed) ring_mask[ring_head] = fz >= 0 ? kMaskFused : kMaskSplit;
+    *fused = fz >= 0;
     TailArgs ta;
     ta.fs = fs;
     ta.status = st_dev;
@@ -651,14 +695,23 @@ int Context::enqueue(FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, co
     ta.q1 = (uint32_t)E.q1;
     const size_t lds = tail_lds_bytes(E.n, E.part_words);
     hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
-    if (timed) record(ev[6], st);
-    timed_valid = timed;
+    if (ev) (void)hipEventRecord(ev[6], st);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
         set_err("kernel launch failed: %s", hipGetErrorString(le));
         return -1;
     }
     return 0;
+}
+
+int Context::enqueue(const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx,
+                     int *st_dev, hipStream_t st, bool timed, bool record_start) {
+    bool fused = false;
+    const int rc = launch_batch(*e, w, fs, B, outp, out_stride, oidx, st_dev, st, timed ? ev : nullptr, record_start,
+                                &fused);
+    if (timed) ring_mask[ring_head] = fused ? kMaskFused : kMaskSplit;
+    timed_valid = timed;
+    return rc;
 }
 
 // Inserts the stage events of a batch into its captured launch chain as event-record nodes,
@@ -714,7 +767,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
                            hipStream_t st) {
     if (graph_mode == 0 || (graph_mode < 0 && !e->use_graphs)) {
         next_set();
-        return enqueue(fs, B, outp, out_stride, oidx, st_dev, st, true);
+        return enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, true);
     }
     const int MT = (int)(round_up(B, 4) / 4);
     const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT), fz = e->fused_for(MT);
@@ -743,7 +796,7 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     }
     hipGraph_t graph = nullptr;
     if (!HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return -1;
-    const int rc = enqueue(fs, B, outp, out_stride, oidx, st_dev, st, false);
+    const int rc = enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, false);
     const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
     if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, rs))) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -830,6 +883,42 @@ bool tail_config_ok(size_t n, uint64_t q1, int q1_bits, size_t row1_off, size_t 
     if (n < 16 || (n & (n - 1))) return false;
     if (q1 >= (1ull << 21) || n % 8 || (row1_off + n * (size_t)q1_bits + 7) / 8 > part_len || q1_bits > 24) return false;
     return true;
+}
+
+// -------------------------------------------------------- allocation-free batches --
+
+// The workspace of alto_answer_device_noalloc: one fixed array on every device the library
+// runs on, held until the process ends.  In host code `scratch` only names the variable; the
+// device address comes from hipGetSymbolAddress, looked up once per device (with that device
+// current) when the first engine on it is created, and only that address is handed to
+// kernels.  Every allocation-free call on a device shares its array.
+constexpr size_t kScratchBytes = (size_t)ALTO_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+int g_scratch_state[kMaxDevices] = {};  // 0 not looked up, 1 found, -1 lookup failed
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+// Looks up the device address of `scratch` on the current device `dev` (once per device).
+void scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_state[dev]) return;
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch[dev] = rc == hipSuccess ? ptr : nullptr;
+    g_scratch_state[dev] = rc == hipSuccess && ptr ? 1 : -1;
+}
+
+void *scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_state[dev] == 1 ? g_scratch[dev] : nullptr;
 }
 
 }  // namespace
@@ -935,6 +1024,7 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
     }
     auto *e = new Engine;
     if (!HIP_OK(hipGetDevice(&e->device))) { delete e; return nullptr; }
+    scratch_setup(e->device);  // device address of the allocation-free entry point's workspace
     e->ld = ld; e->rows = rows; e->cols = cols; e->n = n; e->tg = tg; e->kp = tg * n; e->outputs = cols / n;
     e->part_len = part_len; e->part_words = (part_len + 3) / 4; e->row1_off = row1_off; e->q1 = q1;
     e->q1_bits = q1_bits; e->threads = threads;
@@ -1086,6 +1176,38 @@ int alto_answer_device(void *p, const uint8_t *slots, size_t slot_stride, size_t
     return alto_answer_gather(e->ctx0, slots, slot_stride, nullptr, B, out, out_stride, nullptr, status, stream);
 }
 
+int alto_answer_device_noalloc(void *p, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
+                               size_t out_stride, int *status, void *stream) {
+    auto *e = (Engine *)p;
+    if (!e || !slots || !out || !status || ((uintptr_t)slots & 15) || (slot_stride & 15) ||
+        slot_stride < ALTO_FRAME_OFFSET + e->request_len() || out_stride < e->response_len()) {
+        set_err("alto_answer_device_noalloc: bad arguments");
+        return -1;
+    }
+    if (!B) return 0;
+    if (!e->tail_ok) {
+        set_err("unsupported tail configuration");
+        return -3;
+    }
+    const size_t Bp = round_up(B, 4), need = workspace_bytes(*e, Bp);
+    if (need > kScratchBytes) {
+        set_err("alto_answer_device_noalloc: a batch of %zu queries needs %zu bytes of workspace on this engine, "
+                "more than the %zu of the fixed scratch array",
+                B, need, kScratchBytes);
+        return -1;
+    }
+    void *base = scratch_address(e->device);
+    if (!base) {
+        set_err("alto_answer_device_noalloc: no device address for the scratch array on device %d", e->device);
+        return -1;
+    }
+    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    hipStream_t st = stream ? (hipStream_t)stream : e->ctx0->stream;
+    bool fused = false;
+    return launch_batch(*e, workspace_at(*e, Bp, (uint8_t *)base), FrameSet{slots, slot_stride, nullptr}, B, out,
+                        out_stride, nullptr, status, st, nullptr, false, &fused);
+}
+
 void *alto_context_new(void *p) {
     auto *e = (Engine *)p;
     if (!e) return nullptr;
@@ -1177,7 +1299,7 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
         return 0;
     }
     // the upload is timed as part of the validation stage (ev[0] was recorded before it)
-    if (ctx->enqueue(fs, B, ctx->out, rl, nullptr, ctx->status, st, true, false)) return -1;
+    if (ctx->enqueue(ctx->ws(), fs, B, ctx->out, rl, nullptr, ctx->status, st, true, false)) return -1;
     if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
         !HIP_OK(hipStreamSynchronize(st)))
         return -1;

@@ -115,6 +115,24 @@ int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t
  * (alto_engine_context); more contexts let several batches of one engine be in flight at
  * once (one batch per context at a time).  alto_context_reserve sizes a context's
  * workspace for B queries ahead of time (0, or -4 when device memory is short). */
+/* No counterpart in the reference: alto_answer_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, statuses, returns and results as alto_answer_device, but it
+ * calls no hipMalloc, hipFree or other allocating routine: the batch workspace (limbs,
+ * accumulators, per-query sums) lives in one fixed device array of
+ * ALTO_NOALLOC_SCRATCH_BYTES per device, held until the process ends, and the kernels clear
+ * what must start at zero on every call.  The launches are eager, on `stream` (null: the
+ * engine's own stream), and never captured by the library.  A batch of B queries needs
+ *   W(B) = sum over {4 Bp rows, 4 Bp tg n, 4 Bp cols, 4 Bp cols, 4 Bp, 8 Bp} of each size
+ *          rounded up to 256 bytes,  Bp = B rounded up to a multiple of 4,
+ * and is refused (-1, alto_last_error says why) when W(B) > ALTO_NOALLOC_SCRATCH_BYTES
+ * (B <= 156 at 1 GiB of 32 KB records, 48 at 8 GiB, 4 at 64 GiB).  Every allocation-free call on
+ * one device shares the array, which keeps the last call's contents: calls that overlap on
+ * one device (several streams or threads) race and must be serialised by the caller. */
+#define ALTO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int alto_answer_device_noalloc(void *e, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
+                               size_t out_stride, int *status, void *stream);
+
 void *alto_context_new(void *e);
 void alto_context_free(void *ctx);
 void *alto_engine_context(void *e);
