@@ -141,6 +141,7 @@ def run_batch(db, B, seed):
     (262144, 37, (0, 7), 3),
     (262144, 500, (0, 7), 3),
     (262144, 2048, (0, 7), 3),
+    (2097152, 1, (0, 15), 4),
     (2097152, 128, (0, 15), 4),
     (2097152, 1024, (0, 15), 4),
 ])
@@ -161,6 +162,41 @@ def test_full_size_batch_matches_the_oracle(hip_device, rows, B, parts, seed):
                 got = outs[k, o * inp.part_len:(o + 1) * inp.part_len]
                 assert np.array_equal(got, out_o[k]), f"{rows} records, batch {B}: part {o} of request {c} differs"
         log(f"{rows} records, batch {B}: part {o} of requests {chk} byte-identical ({time.time() - t0:.1f} s oracle)")
+
+
+def test_every_variant_on_long_record_ranges(hip_device):
+    # 4 GiB (131,072 records, 2,048 k-steps): every variant, forced for both contractions,
+    # must give the oracle's bytes with its own tile grid and record-range splits (split
+    # results added with atomics) at a batch of 128
+    from altopir_public_amd import _lib, autotune
+
+    db = FullDb.get(131072)
+    inp = db.inp
+    lib = _lib.load()
+    names = autotune.variant_names()
+    B = 128
+    checked = None
+    for i, name in enumerate(names):
+        for which in (0, 1):
+            assert lib.alto_tune_set(db.eng.ptr, which, B, i) == 0
+        chk, frames, st, outs, want = run_batch(db, B, seed=6)
+        assert list(st) == want, name
+        if checked is None:
+            checked = {}
+            for o in (0, 15):
+                sub = part_inputs(inp, o, db.part(o))
+                out_o = np.zeros((len(chk), sub.response_len), np.uint8)
+                st_o = np.full(len(chk), -1, np.int32)
+                oc.oracle_answer(sub, frames, out_o, st_o)
+                checked[o] = out_o
+        for o, out_o in checked.items():
+            for k, c in enumerate(chk):
+                if want[c] == 0:
+                    got = outs[k, o * inp.part_len:(o + 1) * inp.part_len]
+                    assert np.array_equal(got, out_o[k]), f"variant {name}: part {o} of request {c} differs"
+    for which in (0, 1):
+        lib.alto_tune_set(db.eng.ptr, which, B, -1)
+    log(f"131072 records, batch 128: {len(names)} variants byte-identical on parts 0 and 15")
 
 
 def test_full_size_host_entry_point(hip_device):
