@@ -157,6 +157,27 @@ void launch_dma2(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
                            (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+void prepare_dmab() {
+    constexpr size_t lds = (size_t)(D + 2) * WM * TM * 1024;
+    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+void launch_dmab(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    constexpr size_t lds = (size_t)(D + 2) * WM * TM * 1024;
+    const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
+    if (g.gz > 1)
+        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+}
+
 // Kernel variants of a contraction.  Every variant computes the full result for any number
 // of A tiles (ragged tiles are clamped and masked); they differ in tile shapes and in how
 // operands are staged, which the tuner chooses per batch size.
@@ -232,6 +253,13 @@ const Variant kVariants[] = {
      prepare_dma<4, 8, 4, 2, 4, false, 1, true>, prepare_dma<4, 8, 4, 2, 4, true, 1, true>},
     {"dma8k2p_m8_n6_s4", launch_dma<8, 6, 2, 4, 4, false, 2, true>, launch_dma<4, 8, 4, 2, 4, true, 2, true>,
      prepare_dma<8, 6, 2, 4, 4, false, 2, true>, prepare_dma<4, 8, 4, 2, 4, true, 2, true>},
+    // A blocks through an LDS ring, each wave's B blocks straight into registers (16 x 16 tiles)
+    {"dmab8_m8_n4_u4", launch_dmab<8, 4, 2, 4, 2, 4, false>, launch_dmab<8, 4, 2, 4, 2, 4, true>,
+     prepare_dmab<8, 4, 2, 4, 2, 4, false>, prepare_dmab<8, 4, 2, 4, 2, 4, true>},
+    {"dmab8_m8_n4_u8", launch_dmab<8, 4, 2, 4, 2, 8, false>, launch_dmab<8, 4, 2, 4, 2, 8, true>,
+     prepare_dmab<8, 4, 2, 4, 2, 8, false>, prepare_dmab<8, 4, 2, 4, 2, 8, true>},
+    {"dmab8_m8_n4_u16", launch_dmab<8, 4, 2, 4, 2, 16, false>, launch_dmab<8, 4, 2, 4, 2, 16, true>,
+     prepare_dmab<8, 4, 2, 4, 2, 16, false>, prepare_dmab<8, 4, 2, 4, 2, 16, true>},
     {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
      prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,
@@ -284,6 +312,31 @@ struct FusedRing {
                                (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
     }
 };
+// Both contractions on the register-streaming 16 x 16-tile workgroups.
+template <int TM, int TN, int WM, int WN, int D, int UNR>
+struct FusedRegB {
+    using SA = DmabShape<TM, TN, WM, WN, D, UNR, false, true>;
+    using SN = DmabShape<TM, TN, WM, WN, D, UNR, false, false>;
+    using P = DmabShape<TM, TN, WM, WN, D, UNR, true, false>;
+    static constexpr size_t kLds = SA::kLds;
+    static void prepare() {
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SA, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SN, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+    }
+    static void launch(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya,
+                       const int8_t *u, hipStream_t st) {
+        const int sb = plan_ring<TM, TN, WM, WN>(gs, false), pb = plan_ring<TM, TN, WM, WN>(gp, true);
+        if (gs.gz > 1)
+            hipLaunchKernelGGL((scan_pack_kernel<SA, P>), dim3(sb + pb), dim3(SA::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+        else
+            hipLaunchKernelGGL((scan_pack_kernel<SN, P>), dim3(sb + pb), dim3(SN::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+    }
+};
+using FusedRegB16 = FusedRegB<8, 4, 2, 4, 2, 16>;
 using FusedRing8 = FusedRing<4, 8, 4, 2, 4, 1, 4, 8, 4, 2, 4, 1>;
 using FusedRingWide = FusedRing<8, 6, 2, 4, 4, 2, 4, 8, 4, 2, 4, 1>;
 
@@ -298,6 +351,7 @@ const FusedPair kFused[] = {
     {"stream_m2_n4_u4", "stream_m1_n4_u4", launch_fused_stream<2, 4, 4, 1, 4, 4>, prepare_none},
     {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
     {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
+    {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", FusedRegB16::launch, FusedRegB16::prepare},
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
@@ -307,21 +361,21 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 // shape, whose loads issue in a better order: 2% faster at 1-4 GiB, even at 64 GiB); the
 // scan of 9-32 queries on 8 x 32-tile workgroups (each streamed query row meets twice the
 // database columns of the square tile) and its packing on 8 x 16 tiles; the scan of 33-48
-// queries on 12-row tiles; the 8-wave 16 x 16-tile ring up to 128 queries; above it the
-// scan on 16 x 24 tiles with two k-steps per barrier (it needs enough query rows to fill
-// the chip with its wider tiles).
+// queries on 12-row tiles; above that both contractions on the 16 x 16-tile workgroups that
+// stream the B blocks into registers, 16 k-steps per loop iteration (at batch 128: scan
+// 0.432 against 0.492 ms for the best all-LDS ring at 1 GiB, 30.3 against 30.6 ms at
+// 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB batch 2048 59.2 against 60.1 ms).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
-                     stream4 = variant_index("stream_m4_n4_u2"), dma8 = variant_index("dma8_m4_n8_s4"),
-                     rows8 = variant_index("dma8_m4_n8_w24_s4"), rows8p = variant_index("dma8_m2_n8_s4"),
-                     rows12 = variant_index("dma8_m3_n10_s4"), wide = variant_index("dma8k2_m8_n6_s4");
+                     stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
+                     rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
+                     regb = variant_index("dmab8_m8_n4_u16");
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
-    if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : dma8;
+    if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : regb;
     if (MT <= 8) return rows8;
     if (MT <= 12) return rows12;
-    if (MT <= 32) return dma8;
-    return wide;
+    return regb;
 }
 int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)

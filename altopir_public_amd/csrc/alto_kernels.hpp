@@ -834,6 +834,152 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__re
     }
 }
 
+// One 1 KiB operand block of zeros: the B block of the k-steps that the register-streaming
+// contraction runs past the end of its range (their products are then zero).
+__device__ __align__(16) i32x4 kZeroBlock[64];
+
+// Contraction with the A blocks through LDS and the B blocks straight into registers, for
+// many queries.  Workgroup = WM x WN waves (two per SIMD), wave tile TM A x TN B tiles (its
+// accumulators in ordinary registers), workgroup tile BM = WM * TM A tiles x BN = WN * TN B
+// tiles.  Each k-step the A blocks go from memory into an LDS ring (global_load_lds, shared
+// by the WN waves of a row), while every wave loads its own TN B blocks into a register ring
+// D k-steps ahead (the WM waves of a column load the same blocks; the second request hits the
+// CU's L1).  Per k-step a CU moves BM KiB into LDS and reads WM * WN * TM KiB out of it:
+// with 16 x 16 tiles 80 KiB of LDS traffic against 128 KiB for the all-LDS ring.
+//
+// The A fragments of step k + 1 are read into the registers of step k's fragments right
+// after the MFMAs that use them, so the barrier that publishes step k + 1 comes before step
+// k's matrix work and the MFMAs do not wait on LDS.  Loads are issued in pairs P(t) = (A(t +
+// 1) into the ring, B(t) into the register ring), one pair per step and D pairs ahead; at the
+// top of step k, P(k) must have landed, so the D - 1 younger pairs stay in flight.  Ring
+// slots: S = D + 2 (step k's fragments may still be in flight from LDS, step k + 1 is read, D
+// are loading).  The loop runs whole groups of UNR steps (straight-line code: the compiler
+// keeps the accumulators in place, and its own wait for the B registers is exact except at
+// the top of each group, where it waits for every load): steps past the end of the range
+// reload the last A block into slots no remaining step reads and take their B blocks from
+// kZeroBlock, so they add nothing; all loads are waited for before the workgroup ends.
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+__device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                          const GemmArgs &g, int L) {
+    constexpr int NW = WM * WN;
+    static_assert(NW == 8, "two waves per SIMD");
+    constexpr int BM = WM * TM;
+    static_assert(BM % NW == 0, "A blocks split evenly over the waves");
+    static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    static_assert(D >= 2 && D <= 3, "prefetch distance");
+    static_assert(UNR % D == 0, "whole register rings per loop iteration");
+    constexpr int S = D + 2;
+    constexpr int PA = BM / NW;         // A blocks each wave copies per step
+    constexpr int PAIR = PA + TN;       // vector-memory instructions per step and wave
+    static_assert(D * PAIR < 64, "vmcnt range");
+    constexpr int BS = b_kstride<PACK>();
+    extern __shared__ i32x4 ring[];     // [S][BM][64]
+    const TileMap tm = map_tile_at(L, g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave / WN, wn = wave % WN;
+    const int mt0 = tm.y * BM, bt0 = (tm.x * WN + wn) * TN;
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    const i32x4 *asrc[PA];
+#pragma unroll
+    for (int p = 0; p < PA; p++) asrc[p] = a_src + (size_t)min(mt0 + wave + NW * p, g.MT - 1) * g.KC * 64 + lane;
+    const i32x4 *bsrc[TN];
+#pragma unroll
+    for (int n = 0; n < TN; n++) bsrc[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
+    const i32x4 *zero = kZeroBlock + lane;
+    auto issue_a = [&](int t) {  // A(t) into its slot (t past the range: the last block)
+        i32x4 *dst = ring + (size_t)(((t - kb) % S) * BM) * 64;
+        const size_t kk = (size_t)min(t, ke - 1);
+#pragma unroll
+        for (int p = 0; p < PA; p++)
+            __builtin_amdgcn_global_load_lds((const void *)(asrc[p] + kk * 64), (lds_void *)(dst + (wave + NW * p) * 64),
+                                             16, 0, 0);
+    };
+    auto load_b = [&](int t, i32x4 *b) {  // B(t) (t past the range: zeros)
+        const bool live = t < ke;
+#pragma unroll
+        for (int n = 0; n < TN; n++) b[n] = *(live ? bsrc[n] + (size_t)t * BS : zero);
+    };
+    i32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+    if (ke > kb) {  // an empty record range leaves the accumulators zero
+        i32x4 a[TM];
+        i32x4 breg[D][TN];
+        // prologue: A(kb), then the pairs P(kb) .. P(kb + D - 1), in this order (the wait
+        // counts below and in the loop rely on it)
+        issue_a(kb);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            issue_a(kb + 1 + d);
+            load_b(kb + d, breg[d]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // every prologue load landed: once per workgroup, and the loop then starts from the
+        // same pending loads on every entry (the compiler's own wait counts for the B
+        // registers stay exact instead of draining every iteration)
+        wait_vmcnt<0>();
+        stage_barrier();
+        {
+            const i32x4 *cur = ring + (size_t)(wm * TM) * 64 + lane;
+#pragma unroll
+            for (int m = 0; m < TM; m++) a[m] = cur[m * 64];
+        }
+        const int kend = kb + (ke - kb + UNR - 1) / UNR * UNR;
+        for (int k0 = kb; k0 < kend; k0 += UNR) {
+#pragma unroll
+            for (int jj = 0; jj < UNR; jj++) {
+                const int j = jj % D;
+                const int k = k0 + jj;
+                wait_vmcnt<(D - 1) * PAIR>();  // P(k): A(k + 1) and B(k)
+                stage_barrier();               // A(k + 1) visible; slot of step k - 1 free
+                const i32x4 *nxt = ring + (size_t)(((k + 1 - kb) % S) * BM + wm * TM) * 64 + lane;
+                // kept in this order (scheduling barriers): hoisting a fragment read or the B
+                // loads above the MFMAs that free their registers would double the live
+                // operands
+#pragma unroll
+                for (int m = 0; m < TM; m++) {
+#pragma unroll
+                    for (int n = 0; n < TN; n++)
+                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], breg[j][n], acc[m][n], 0, 0, 0);
+                    a[m] = nxt[m * 64];  // step k + 1's fragment (past the range: unused)
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                issue_a(k + 1 + D);  // into the slot of step k - 1
+                load_b(k + D, breg[j]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        wait_vmcnt<0>();  // loads past the range land before the workgroup's LDS is released
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int rt = mt0 + wm * TM + m;
+        if (rt >= g.MT) continue;
+        if (PACK) {
+#pragma unroll
+            for (int n = 0; n < TN; n += 4) {
+                const int t = bt0 + n;
+                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < TN; n++) {
+                const int t = bt0 + n;
+                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+            }
+        }
+    }
+}
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(64 * WM * WN, 1) void dmab_kernel(const i32x4 *__restrict__ a_src,
+                                                               const i32x4 *__restrict__ b_src, GemmArgs g) {
+    dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a_src, b_src, g, blockIdx.x);
+}
+
 // ---------------------------------------------------- both contractions in one launch --
 
 // Tile shapes as types, so that one kernel can run the tiles of two contractions.
@@ -851,6 +997,15 @@ struct DmaShape {
     static constexpr size_t kLds = (size_t)S * (WM * TM + WN * TN) * kBlockBytes;
     static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
         dma_tile<TM, TN, WM, WN, S, PACK, ATOMIC, KPB>(a, b, g, L);
+    }
+};
+
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+struct DmabShape {
+    static constexpr int kThreads = 64 * WM * WN;
+    static constexpr size_t kLds = (size_t)(D + 2) * WM * TM * kBlockBytes;
+    static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
+        dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a, b, g, L);
     }
 };
 

@@ -113,8 +113,8 @@ def test_multi_block_and_chunked_batches_match_the_oracle(hip_device, B):
 @pytest.mark.parametrize("B", [3, 9, 24, 33, 48, 49, 129])
 def test_built_in_shapes_at_every_bucket_edge_match_the_oracle(hip_device, B):
     # the built-in shape choice changes at 4, 8, 32 and 48 queries (streaming, 8-row ring,
-    # 12-row ring, 16-row ring) and at 128 (16 x 24 tiles); each side of those edges, with the
-    # default packing shape of the same batch size
+    # 12-row ring, then the 16-row workgroups that stream B blocks into registers); each side
+    # of those edges, with the default packing shape of the same batch size
     inp = _block_inputs()
     reqs = workload.make_requests(B, inp.rows, inp.kp, seed=300 + B)
     g, o = run_both(inp, reqs)
@@ -201,6 +201,30 @@ def test_every_kernel_variant_gives_the_same_bytes(hip_device, B):
     lib = _lib.load()
     names = autotune.variant_names()
     for i, name in enumerate(names):
+        for which in (0, 1):
+            assert lib.alto_tune_set(eng.ptr, which, B, i) == 0
+        out = np.zeros((B, rl), np.uint8)
+        st = np.full(B, -1, np.int32)
+        assert eng.answer(reqs, out, st)[0] == B
+        bad = [c for c in range(B) if not np.array_equal(out[c], out_o[c])]
+        assert not bad, f"variant {name}: responses {bad[:8]} differ"
+    eng.close()
+
+
+@pytest.mark.parametrize("B", [5, 72])
+def test_every_kernel_variant_on_an_odd_number_of_k_steps(hip_device, B):
+    # 2,368 records = 37 k-steps of 64 (a prime): record-range splits and the unrolled k-loops
+    # of the contraction kernels end on partial groups of steps
+    from altopir_public_amd import _lib, autotune
+
+    inp = workload.make_inputs(64 * 37, 4096, n=2048, tg=4, seed=37, prefix="random")
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=600 + B)
+    rl = inp.response_len
+    out_o = np.zeros((B, rl), np.uint8)
+    oc.oracle_answer(inp, reqs, out_o, np.full(B, -1, np.int32))
+    eng = Engine.from_inputs(inp)
+    lib = _lib.load()
+    for i, name in enumerate(autotune.variant_names()):
         for which in (0, 1):
             assert lib.alto_tune_set(eng.ptr, which, B, i) == 0
         out = np.zeros((B, rl), np.uint8)
@@ -393,8 +417,8 @@ def test_tuner_records_a_variant_per_bucket_and_keeps_the_bytes(hip_device, smal
 @pytest.mark.parametrize("B", [1, 6, 12, 40, 60, 136])
 def test_scan_and_packing_in_one_launch_give_the_same_bytes(hip_device, small_inputs, B):
     # the built-in shape pairs run both contractions in one launch (B = 1 and 6 streaming,
-    # 60 the 16x16 ring, 136 the 16x24 ring; 12 and 40 scan on 8- and 12-row tiles, which
-    # have no fused pair): fused, unfused and oracle answers agree.  By default only batches
+    # 60 and 136 the register-streaming 16 x 16 tiles; 12 and 40 scan on 8- and 12-row tiles,
+    # which have no fused pair): fused, unfused and oracle answers agree.  By default only batches
     # of one A tile (1-4 queries) run fused (on this database of 2048 <= 4 KP records).
     fusable = B not in (12, 40)
     inp = small_inputs
