@@ -18,7 +18,7 @@ SYMBOLS = (
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
     "alto_engine_new_device", "alto_engine_new_records", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
     "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
-    "alto_context_set_graphs", "alto_set_fusion", "alto_fused", "alto_set_graph_timing",
+    "alto_context_set_graphs", "alto_context_graphs", "alto_set_fusion", "alto_fused", "alto_set_graph_timing",
     "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
@@ -102,6 +102,8 @@ def load() -> C.CDLL:
     lib.alto_fused.argtypes = [P, SZ]
     lib.alto_context_set_graphs.restype = C.c_int
     lib.alto_context_set_graphs.argtypes = [P, C.c_int]
+    lib.alto_context_graphs.restype = C.c_int
+    lib.alto_context_graphs.argtypes = [P]
     lib.alto_answer_gather.restype = C.c_int
     lib.alto_answer_gather.argtypes = [P, P, SZ, P, SZ, P, SZ, P, P, P]
     lib.alto_context_timings.restype = C.c_int

@@ -510,9 +510,14 @@ struct Context {
     std::vector<int> hstatus;
     // captured batches, replayed when the same batch (pointers, size, stream, kernel
     // variants) comes again
+    // A bucket graph (bucket >= 0) is captured for the largest batch of its bucket (B = 4 *
+    // bucket_top) behind a batch_dims_kernel node, whose arguments every replay sets to the
+    // batch's own {B, MT}: the kernels read them and skip the rows past the batch, so one
+    // graph answers every batch size of the bucket.  Batches beyond kMaxBucketTiles tiles,
+    // and timed replays, are captured per exact size (bucket = -1).
     struct GraphEntry {
         FrameSet fs;
-        size_t B;
+        size_t B;   // batch the chain is sized for
         uint8_t *out;
         size_t out_stride;
         const int *oidx;
@@ -520,6 +525,10 @@ struct Context {
         hipStream_t st;
         int vs, vp, fz;
         int ring;  // timing event set recorded by the graph, -1 for none
+        int bucket;
+        hipGraph_t graph;            // kept for the node updates of a bucket graph
+        hipGraphNode_t dims_node;
+        hipKernelNodeParams dims_params;
         hipGraphExec_t exec;
         uint64_t used;
     };
@@ -527,10 +536,13 @@ struct Context {
     int graph_mode = -1;  // -1: the engine's setting, 0: eager, 1: capture and replay
     uint64_t tick = 0;
     static constexpr size_t kMaxGraphs = 64;
+    static constexpr int kMaxBucketTiles = 512;  // bucket graphs up to 2048 queries
+    int *dims = nullptr;  // device {B, MT} of the batch a bucket graph answers
 
     bool init(Engine *eng) {
         e = eng;
-        bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) &&
+                  HIP_OK(hipMalloc(&dims, 2 * sizeof(int)));
         for (auto &set : ring)
             // device-scope release: the stage events only timestamp the stream (nothing reads
             // device memory on the host through them), so recording one need not write the L2
@@ -575,8 +587,12 @@ struct Context {
         for (int k = 0; k < kRing; k++) ok = fold(k) && ok;
         return ok;
     }
+    static void destroy(GraphEntry &g) {
+        (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     void drop_graphs() {
-        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (auto &g : graphs) destroy(g);
         graphs.clear();
     }
     void release_buffers() {
@@ -597,6 +613,7 @@ struct Context {
         for (auto &set : ring)
             for (auto &x : set)
                 if (x) (void)hipEventDestroy(x);
+        if (dims) (void)hipFree(dims);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -665,14 +682,15 @@ Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base) {
 // the two contractions (or both in one launch), tail.  ev: the stage events to record at the
 // launch boundaries (null: none); *fused is set when both contractions ran in one launch.
 int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, size_t out_stride,
-                 const int *oidx, int *st_dev, hipStream_t st, hipEvent_t *ev, bool record_start, bool *fused) {
+                 const int *oidx, int *st_dev, hipStream_t st, hipEvent_t *ev, bool record_start, bool *fused,
+                 const int *dims = nullptr) {
     int8_t *const qa = w.qa, *const ya = w.ya;
     uint32_t *const acc = w.acc, *const beta = w.beta, *const qsum = w.qsum;
     unsigned long long *const ysum = w.ysum;
     const size_t Bp = round_up(B, 4), MT = Bp / 4;
     if (ev && record_start) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(E.kp)), dim3(256), 0, st, fs, st_dev,
-                       (uint32_t)E.kp, qsum, ysum);
+                       (uint32_t)E.kp, qsum, ysum, dims);
     if (ev) (void)hipEventRecord(ev[1], st);
     GemmArgs gs{};  // scan: query limbs x database -> acc
     gs.KC = (int)E.KC;
@@ -680,6 +698,7 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     gs.NT = (int)E.CT;
     gs.C = (int)E.cols;
     gs.out = acc;
+    gs.dims = dims;
     GemmArgs gp{};  // packing: key limbs x packing matrix -> beta
     gp.KC = (int)E.KCp;
     gp.MT = (int)MT;
@@ -690,6 +709,7 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     gp.ysum = ysum;
     gp.k128e = E.k128e;
     gp.ycorr0 = E.ycorr0;
+    gp.dims = dims;
     // one launch prepares both limb sets and clears the scan accumulators
     PrepArgs pa;
     pa.fs = fs;
@@ -709,6 +729,7 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     pa.px = (unsigned)((E.KCp * 4 + 255) / 256);
     pa.py = (unsigned)Bp;
     pa.nfill = (unsigned)std::min<size_t>(1024, (pa.acc4 + 1023) / 1024);
+    pa.dims = dims;
     const unsigned prep_blocks = pa.sx * pa.sy + pa.px * pa.py + pa.nfill;
     hipLaunchKernelGGL(prep_kernel, dim3(prep_blocks), dim3(256), 0, st, pa);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -747,6 +768,7 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     ta.row1_off = (long long)E.row1_off;
     ta.q1_bits = E.q1_bits;
     ta.q1 = (uint32_t)E.q1;
+    ta.dims = dims;
     const size_t lds = tail_lds_bytes(E.n, E.part_words);
     hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
     if (ev) (void)hipEventRecord(ev[6], st);
@@ -824,44 +846,82 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
         return enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, true);
     }
     const int MT = (int)(round_up(B, 4) / 4);
-    const int vs = e->variant_for(0, MT), vp = e->variant_for(1, MT), fz = e->fused_for(MT);
     // timed replays: the batch is captured once per timing event set, each graph recording
     // its set's events, and the replays cycle through kGraphRing sets
     const bool tg = e->time_graphs;
+    const int bk = bucket_of(MT);
+    const bool bucketed = !tg && bucket_top(bk) <= kMaxBucketTiles;
+    const int MTg = bucketed ? (int)bucket_top(bk) : MT;  // A tiles the launch chain is sized for
+    const size_t Bg = bucketed ? 4 * (size_t)MTg : B;
+    if (bucketed && !reserve(Bg, false)) return -4;
+    // the built-in policy and the tuning table choose per bucket, so a bucket graph's variants
+    // are those of every batch of the bucket
+    const int vs = e->variant_for(0, MTg), vp = e->variant_for(1, MTg), fz = e->fused_for(MTg);
     int rs = -1;
     if (tg) {
         next_set(kGraphRing);
         rs = ring_head;
     }
     timed_valid = tg;
+    auto set_dims = [&](GraphEntry &g) {
+        int *d = dims;
+        int b = (int)B, mt = MT;
+        void *args[] = {&d, &b, &mt};
+        hipKernelNodeParams kp = g.dims_params;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        return HIP_OK(hipGraphExecKernelNodeSetParams(g.exec, g.dims_node, &kp));
+    };
     for (auto &g : graphs)
-        if (g.fs.slots == fs.slots && g.fs.stride == fs.stride && g.fs.sidx == fs.sidx && g.B == B && g.out == outp &&
+        if (g.fs.slots == fs.slots && g.fs.stride == fs.stride && g.fs.sidx == fs.sidx && g.B == Bg && g.out == outp &&
             g.out_stride == out_stride && g.oidx == oidx && g.status == st_dev && g.st == st && g.vs == vs &&
-            g.vp == vp && g.fz == fz && g.ring == rs) {
+            g.vp == vp && g.fz == fz && g.ring == rs && g.bucket == (bucketed ? bk : -1)) {
             g.used = ++tick;
             if (rs >= 0) ring_mask[rs] = g.fz >= 0 ? kMaskFused : kMaskSplit;
+            if (bucketed && !set_dims(g)) return -1;
             return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
         }
     if (graphs.size() >= kMaxGraphs) {
         auto lru = std::min_element(graphs.begin(), graphs.end(),
                                     [](const GraphEntry &a, const GraphEntry &b) { return a.used < b.used; });
-        (void)hipGraphExecDestroy(lru->exec);
+        destroy(*lru);
         graphs.erase(lru);
     }
     hipGraph_t graph = nullptr;
     if (!HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return -1;
-    const int rc = enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, false);
+    int rc;
+    if (bucketed) {
+        hipLaunchKernelGGL(batch_dims_kernel, dim3(1), dim3(64), 0, st, dims, (int)Bg, MTg);
+        bool fused = false;
+        rc = launch_batch(*e, ws(), fs, Bg, outp, out_stride, oidx, st_dev, st, nullptr, false, &fused, dims);
+    } else {
+        rc = enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, false);
+    }
     const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
     if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, rs))) {
         if (graph) (void)hipGraphDestroy(graph);
         return -1;
     }
-    hipGraphExec_t exec = nullptr;
-    const bool inst = HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
+    GraphEntry g{fs, Bg, outp, out_stride, oidx, st_dev, st, vs, vp, fz, rs, bucketed ? bk : -1, nullptr, nullptr, {},
+                 nullptr, 0};
+    if (bucketed) {
+        // the chain's root is the dims node
+        size_t nroot = 1;
+        if (!HIP_OK(hipGraphGetRootNodes(graph, &g.dims_node, &nroot)) || nroot != 1 ||
+            !HIP_OK(hipGraphKernelNodeGetParams(g.dims_node, &g.dims_params))) {
+            set_err("bucket graph: no dims node at the root of the captured chain");
+            (void)hipGraphDestroy(graph);
+            return -1;
+        }
+    }
+    const bool inst = HIP_OK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    if (bucketed && inst) g.graph = graph;  // the dims node's graph stays alive for the updates
+    else (void)hipGraphDestroy(graph);
     if (!inst) return -1;
-    graphs.push_back({fs, B, outp, out_stride, oidx, st_dev, st, vs, vp, fz, rs, exec, ++tick});
-    return HIP_OK(hipGraphLaunch(exec, st)) ? 0 : -1;
+    g.used = ++tick;
+    graphs.push_back(g);
+    if (bucketed && !set_dims(graphs.back())) return -1;
+    return HIP_OK(hipGraphLaunch(g.exec, st)) ? 0 : -1;
 }
 
 // out[i * bc + c] += acc[c][i] + 128 * sum(b_c)   (the scan's offset correction, mod 2^32)
@@ -1290,6 +1350,8 @@ int alto_context_set_graphs(void *c, int mode) {
     return 0;
 }
 
+int alto_context_graphs(const void *c) { return c ? (int)((const Context *)c)->graphs.size() : -1; }
+
 int alto_context_reserve(void *c, size_t B) {
     auto *ctx = (Context *)c;
     if (!ctx) return -1;
@@ -1343,8 +1405,8 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     if (!ok) return -1;
     if (!e->tail_ok) {
         // validation only: the reference fails the whole batch with -3 once any request is valid
-        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(e->kp)), dim3(256), 0, st, fs, ctx->status, (uint32_t)e->kp,
-                           ctx->qsum, ctx->ysum);
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)B, validate_slices(e->kp)), dim3(256), 0, st, fs, ctx->status,
+                           (uint32_t)e->kp, ctx->qsum, ctx->ysum, (const int *)nullptr);
         if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
             !HIP_OK(hipStreamSynchronize(st)))
             return -1;

@@ -209,11 +209,15 @@ __device__ __forceinline__ uint32_t ld32_bytes(const uint8_t *p) {
 // turns a 0 into 3 (compare-and-swap, so a status set on entry stands).  A workgroup that
 // starts after another one's 3 returns at once.  Workgroup (c, 0) also clears the per-query
 // sums that the prep kernel accumulates.
+// dims: the batch dimensions {B, MT} in device memory when the launch is sized for a larger
+// batch of the same bucket (workgroups past B return at once), else null.
 __global__ __launch_bounds__(256) void validate_kernel(FrameSet fs, int *__restrict__ status, uint32_t kp,
                                                        uint32_t *__restrict__ qsum,
-                                                       unsigned long long *__restrict__ ysum) {
+                                                       unsigned long long *__restrict__ ysum,
+                                                       const int *__restrict__ dims) {
     __shared__ int st;
     const size_t c = blockIdx.x;
+    if (dims && (int)c >= dims[0]) return;
     const unsigned s = blockIdx.y, S = gridDim.y;
     if (threadIdx.x == 0) {
         if (s == 0) {
@@ -337,21 +341,32 @@ struct PrepArgs {
     uint4 *acc;
     size_t acc4;
     unsigned sx, sy, px, py, nfill;
+    const int *dims;  // {B, MT} in device memory (launch sized for the batch's bucket), or null
 };
 __global__ __launch_bounds__(256) void prep_kernel(PrepArgs p) {
     unsigned b = blockIdx.x;
+    const int B = p.dims ? p.dims[0] : p.B;  // rows from B on get zero limbs
     if (b < p.sx * p.sy) {
-        limb_prep_block<true>(p.fs, p.scan_off, p.status, p.B, p.KC, p.qa, p.qsum, p.ysum, b % p.sx, b / p.sx);
+        limb_prep_block<true>(p.fs, p.scan_off, p.status, B, p.KC, p.qa, p.qsum, p.ysum, b % p.sx, b / p.sx);
         return;
     }
     b -= p.sx * p.sy;
     if (b < p.px * p.py) {
-        limb_prep_block<false>(p.fs, 0, p.status, p.B, p.KCp, p.ya, p.qsum, p.ysum, b % p.px, b / p.px);
+        limb_prep_block<false>(p.fs, 0, p.status, B, p.KCp, p.ya, p.qsum, p.ysum, b % p.px, b / p.px);
         return;
     }
     b -= p.px * p.py;
     const uint4 z = make_uint4(0, 0, 0, 0);
     for (size_t i = (size_t)b * blockDim.x + threadIdx.x; i < p.acc4; i += (size_t)p.nfill * blockDim.x) p.acc[i] = z;
+}
+
+// Writes the dimensions {B, MT} of the batch a bucket-sized launch chain answers: the first
+// node of a captured chain, whose arguments each replay sets to the batch's own.
+__global__ void batch_dims_kernel(int *__restrict__ dims, int B, int MT) {
+    if (threadIdx.x == 0) {
+        dims[0] = B;
+        dims[1] = MT;
+    }
 }
 
 // ------------------------------------------------------------------- contractions ----
@@ -387,7 +402,12 @@ struct GemmArgs {
     const uint32_t *ucorr;              // pack: column corrections
     const unsigned long long *ysum;     // pack: per-query key sums
     uint32_t k128e, ycorr0;
+    // batch dimensions in device memory {B, MT} (a launch sized for a larger batch of the same
+    // bucket, from a captured graph), or null: MT above is the batch's own
+    const int *dims;
 };
+// A tiles of the batch: from the device dimensions when given, else the launch's own MT.
+__device__ __forceinline__ int batch_tiles(const GemmArgs &g) { return g.dims ? g.dims[1] : g.MT; }
 
 // B operand block of tile t at k-step k (i32x4 units).  Scan: database tile t.  Pack: tile
 // t = 4 * column tile + plane j of the packing matrix.
@@ -444,12 +464,14 @@ __device__ __forceinline__ void stream_tile(const i32x4 *__restrict__ a_src, con
     if (!tm.valid) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int bt0 = (tm.x * 4 + wave) * NBT, mt0 = tm.y * MB;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
     constexpr int BS = b_kstride<PACK>();
     const i32x4 *ap[MB];
     const i32x4 *bp[NBT];
 #pragma unroll
-    for (int m = 0; m < MB; m++) ap[m] = a_src + (size_t)min(mt0 + m, g.MT - 1) * g.KC * 64 + lane;
+    for (int m = 0; m < MB; m++) ap[m] = a_src + (size_t)min(mt0 + m, MT - 1) * g.KC * 64 + lane;
 #pragma unroll
     for (int n = 0; n < NBT; n++) bp[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
     i32x4 acc[MB][NBT];
@@ -488,7 +510,7 @@ __device__ __forceinline__ void stream_tile(const i32x4 *__restrict__ a_src, con
     }
 #pragma unroll
     for (int m = 0; m < MB; m++) {
-        if (mt0 + m >= g.MT) continue;
+        if (mt0 + m >= MT) continue;
         if (PACK) {
 #pragma unroll
             for (int n = 0; n < NBT; n += 4)
@@ -526,6 +548,8 @@ __global__ __launch_bounds__(256, 1) void lds_kernel(const i32x4 *__restrict__ a
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave / WN, wn = wave % WN;
     const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
     // this wave copies blocks wave, wave + 4, ... of the workgroup's NBLK
     const i32x4 *src[PW];
@@ -534,7 +558,7 @@ __global__ __launch_bounds__(256, 1) void lds_kernel(const i32x4 *__restrict__ a
     for (int p = 0; p < PW; p++) {
         const int blk = wave + 4 * p;
         if (blk < BM) {
-            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            src[p] = a_src + (size_t)min(mt0 + blk, MT - 1) * g.KC * 64 + lane;
             kst[p] = 64;
         } else {
             src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
@@ -580,7 +604,7 @@ __global__ __launch_bounds__(256, 1) void lds_kernel(const i32x4 *__restrict__ a
 #pragma unroll
     for (int m = 0; m < TM; m++) {
         const int rt = mt0 + wm * TM + m;
-        if (rt >= g.MT) continue;
+        if (rt >= MT) continue;
         if (PACK) {
 #pragma unroll
             for (int n = 0; n < TN; n += 4) {
@@ -640,6 +664,8 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave / WN, wn = wave % WN;
     const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
     const i32x4 *src[PW];
     int kst[PW];
@@ -647,7 +673,7 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
     for (int p = 0; p < PW; p++) {
         const int blk = wave + NW * p;
         if (blk < BM) {
-            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            src[p] = a_src + (size_t)min(mt0 + blk, MT - 1) * g.KC * 64 + lane;
             kst[p] = 64;
         } else {
             src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
@@ -705,7 +731,7 @@ __device__ __forceinline__ void dma_tile(const i32x4 *__restrict__ a_src, const 
 #pragma unroll
     for (int m = 0; m < TM; m++) {
         const int rt = mt0 + wm * TM + m;
-        if (rt >= g.MT) continue;
+        if (rt >= MT) continue;
         if (PACK) {
 #pragma unroll
             for (int n = 0; n < TN; n += 4) {
@@ -749,6 +775,8 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__re
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave / WN, wn = wave % WN;
     const int mt0 = tm.y * BM, bt0 = tm.x * BN;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
     const i32x4 *src[PW];
     int kst[PW];
@@ -756,7 +784,7 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__re
     for (int p = 0; p < PW; p++) {
         const int blk = wave + NW * p;
         if (blk < BM) {
-            src[p] = a_src + (size_t)min(mt0 + blk, g.MT - 1) * g.KC * 64 + lane;
+            src[p] = a_src + (size_t)min(mt0 + blk, MT - 1) * g.KC * 64 + lane;
             kst[p] = 64;
         } else {
             src[p] = b_block<PACK>(b_src, min(bt0 + blk - BM, g.NT - 1), g.KC) + lane;
@@ -817,7 +845,7 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dma2_kernel(const i32x4 *__re
 #pragma unroll
     for (int m = 0; m < TM; m++) {
         const int rt = mt0 + wm * TM + m;
-        if (rt >= g.MT) continue;
+        if (rt >= MT) continue;
         if (PACK) {
 #pragma unroll
             for (int n = 0; n < TN; n += 4) {
@@ -879,10 +907,12 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave / WN, wn = wave % WN;
     const int mt0 = tm.y * BM, bt0 = (tm.x * WN + wn) * TN;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
     const i32x4 *asrc[PA];
 #pragma unroll
-    for (int p = 0; p < PA; p++) asrc[p] = a_src + (size_t)min(mt0 + wave + NW * p, g.MT - 1) * g.KC * 64 + lane;
+    for (int p = 0; p < PA; p++) asrc[p] = a_src + (size_t)min(mt0 + wave + NW * p, MT - 1) * g.KC * 64 + lane;
     const i32x4 *bsrc[TN];
 #pragma unroll
     for (int n = 0; n < TN; n++) bsrc[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
@@ -958,7 +988,7 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
 #pragma unroll
     for (int m = 0; m < TM; m++) {
         const int rt = mt0 + wm * TM + m;
-        if (rt >= g.MT) continue;
+        if (rt >= MT) continue;
         if (PACK) {
 #pragma unroll
             for (int n = 0; n < TN; n += 4) {
@@ -1044,6 +1074,7 @@ struct TailArgs {
     long long row1_off;
     int q1_bits;
     uint32_t q1;
+    const int *dims;        // {B, MT} in device memory (launch sized for the batch's bucket), or null
 };
 
 // trunc(x / Q) for |x| < 2^53, exactly: a double-precision estimate (off by at most one
@@ -1104,6 +1135,7 @@ __host__ __device__ constexpr size_t tail_lds_bytes(size_t n, size_t part_words)
 __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
     extern __shared__ uint32_t lds[];
     const int o = blockIdx.x, c = blockIdx.y;
+    if (p.dims && c >= p.dims[0]) return;
     if (p.status[c] != 0) return;
     const int n = p.n, logn = p.logn, groups = n >> 3;
     uint32_t *a = lds;

@@ -140,6 +140,11 @@ int alto_context_reserve(void *ctx, size_t B);
 /* Graph mode of one context: 1 capture and replay, 0 eager (drops its graphs), -1 follow
  * the engine's alto_set_graphs (the default).  Returns 0, -1 on bad arguments. */
 int alto_context_set_graphs(void *ctx, int mode);
+/* Number of launch chains the context holds captured.  With graphs on, a batch of up to 2048
+ * queries replays the graph of its batch-size bucket (captured once, for the bucket's largest
+ * batch, and told the batch's own size at each replay); larger batches and timed replays are
+ * captured per exact size.  Returns -1 for a null context. */
+int alto_context_graphs(const void *ctx);
 
 /* alto_answer_device on a context, with frames and outputs gathered through optional
  * device index arrays: frame c at slots + slot_index[c] * slot_stride + ALTO_FRAME_OFFSET,

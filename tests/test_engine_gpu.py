@@ -235,6 +235,55 @@ def test_every_kernel_variant_on_an_odd_number_of_k_steps(hip_device, B):
     eng.close()
 
 
+def test_one_graph_answers_every_batch_size_of_its_bucket(hip_device, small_inputs):
+    # batches of 33-48 queries (12 A tiles at most) share one captured chain, sized for 48 and
+    # told each batch's size at replay: every answer equals the eager one, the rows past the
+    # batch are never written, and the context holds one graph per bucket
+    import torch
+
+    from altopir_public_amd import _lib
+
+    lib = _lib.load()
+    inp = small_inputs
+    L, rl = inp.request_len, inp.response_len
+    stride = (7 + L + 255) // 256 * 256
+    eng = Engine.from_inputs(inp)
+    ctx = lib.alto_engine_context(eng.ptr)
+    Bmax = 48
+    slots = workload.make_requests_device(Bmax, inp.rows, inp.kp, seed=811, stride=stride)
+    slots[5, 7 + 9 + 4 * 77:7 + 9 + 4 * 78] = torch.tensor([255, 255, 255, 255], dtype=torch.uint8,
+                                                           device=slots.device)  # key >= Q: status 3
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run(B, graphs):
+        eng.set_graphs(graphs)
+        out = torch.full((Bmax, rl), 0xA5, dtype=torch.uint8, device="cuda")
+        st = torch.zeros(Bmax, dtype=torch.int32, device="cuda")
+        assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, st.data_ptr(), stream) == 0
+        torch.cuda.synchronize()
+        return out.cpu().numpy(), st.cpu().numpy()
+
+    for B in (37, 48, 33, 44, 40, 37):
+        want_out, want_st = run(B, False)
+        got_out, got_st = run(B, True)
+        assert list(got_st[:B]) == list(want_st[:B]) and got_st[5] == 3 and (got_st[B:] == 0).all()
+        assert np.array_equal(got_out[:B], want_out[:B]), f"batch {B}: replay differs from the eager batch"
+        assert (got_out[B:] == 0xA5).all(), f"batch {B}: rows past the batch were written"
+        assert lib.alto_context_graphs(ctx) == 1, "batches of one bucket captured more than one graph"
+    frames = [slots[c, 7:7 + L].cpu().numpy() for c in range(37)]
+    out_o = np.zeros((37, rl), np.uint8)
+    st_o = np.full(37, -1, np.int32)
+    oc.oracle_answer(inp, frames, out_o, st_o)
+    got_out, got_st = run(37, True)
+    assert list(got_st[:37]) == list(st_o)
+    for c in range(37):
+        if st_o[c] == 0:
+            assert np.array_equal(got_out[c], out_o[c])
+    run(3, True)  # another bucket: a second graph
+    assert lib.alto_context_graphs(ctx) == 2
+    eng.close()
+
+
 def test_graph_replays_match_eager_batches(hip_device, small_inputs):
     # The device entry point captures a batch into a HIP graph and replays it for the same
     # pointers; every replay must equal an eager run of the same frames, with other eager
