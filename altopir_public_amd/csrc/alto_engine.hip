@@ -1114,6 +1114,58 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
     });
 }
 
+int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *const *bold_t_bar,
+                    const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg, size_t n, size_t rho,
+                    uint64_t qmod, int threads) {
+    (void)threads;
+    const size_t K = tg * n, cols = rho * n, per = num_iter * tg * n;
+    if (!ub || !bold_t || !bold_t_bar || !tables || !gen_pows || !num_iter || !tg || !n || !rho || !qmod || K % 64 ||
+        cols % 32 || K * 8 > kMaxLds || cols > (size_t)INT32_MAX || num_iter * tg > (size_t)INT32_MAX) {
+        set_err("alto_pack_build: unsupported shape (needs tg n %% 64 == 0, rho n %% 32 == 0, tg n <= %zu)", kMaxLds / 8);
+        return -1;
+    }
+    for (size_t o = 0; o < rho; o++)
+        if (!bold_t[o] || !bold_t_bar[o]) {
+            set_err("alto_pack_build: null bold_t / bold_t_bar");
+            return -1;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return -1;
+    }
+    uint64_t *dbt = nullptr, *dbb = nullptr;
+    uint32_t *dtab = nullptr, *dpow = nullptr;
+    int8_t *dub = nullptr;
+    hipStream_t st = nullptr;
+    const size_t ubytes = 4 * K * cols;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+              HIP_OK(hipMalloc(&dbt, rho * per * 8)) && HIP_OK(hipMalloc(&dbb, rho * per * 8)) &&
+              HIP_OK(hipMalloc(&dtab, n * n * 4)) && HIP_OK(hipMalloc(&dpow, num_iter * 4)) &&
+              HIP_OK(hipMalloc(&dub, ubytes));
+    for (size_t o = 0; ok && o < rho; o++)
+        ok = HIP_OK(hipMemcpyAsync(dbt + o * per, bold_t[o], per * 8, hipMemcpyHostToDevice, st)) &&
+             HIP_OK(hipMemcpyAsync(dbb + o * per, bold_t_bar[o], per * 8, hipMemcpyHostToDevice, st));
+    ok = ok && HIP_OK(hipMemcpyAsync(dtab, tables, n * n * 4, hipMemcpyHostToDevice, st)) &&
+         HIP_OK(hipMemcpyAsync(dpow, gen_pows, num_iter * 4, hipMemcpyHostToDevice, st));
+    if (ok) {
+        const size_t lds = K * 8;
+        ok = HIP_OK(hipFuncSetAttribute((const void *)pack_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+        if (ok) {
+            hipLaunchKernelGGL(pack_build_kernel, dim3((unsigned)cols), dim3(256), lds, st, (const uint64_t *)dbt,
+                               (const uint64_t *)dbb, per, (const uint32_t *)dtab, (const uint32_t *)dpow,
+                               (int)num_iter, (int)tg, (int)n, (int)cols, qmod, dub);
+            ok = HIP_OK(hipGetLastError());
+        }
+    }
+    ok = ok && HIP_OK(hipMemcpyAsync(ub, dub, ubytes, hipMemcpyDeviceToHost, st)) && HIP_OK(hipStreamSynchronize(st));
+    for (void *p : {(void *)dbt, (void *)dbb, (void *)dtab, (void *)dpow, (void *)dub})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
 static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_rows, const uint8_t *db_tiled, size_t ld,
                            size_t rows,
                            size_t cols, const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,

@@ -154,6 +154,56 @@ __global__ void tile_u_kernel(const int8_t *__restrict__ plane, int8_t *__restri
     *(uint4 *)(dst + ((ct * KCp + kc) * 4 + j) * kBlockBytes + lane * 16) = v;
 }
 
+// Packing-matrix build (src/amx_kernels.cpp:311-341) on the device, one workgroup per output
+// column oz = (o, z) of the rho * n columns:
+//   col[k n + zp(i, z)] += bold_t[o][(i tg + k) n + z],  col[k n + zb(i, z)] += bold_t_bar[...]
+// for i < num_iter, k < tg, with zp = tables[((g - 1) / 2) n + z], zb = tables[((2n - g - 1) / 2) n
+// + z], g = gen_pows[i]; sums wrap mod 2^64 as the reference's u64 column does (LDS atomics), then
+// each entry is reduced mod q and its four bytes are stored as byte - 128 in the reference's
+// plane layout (byte plane j of U[kk][oz] at ((j (CT/2) + ct/2) KC + kc) 2048 + (ct % 2) 1024 +
+// r 64 + 4 nn + t, ct = oz / 16, nn = oz % 16, kc = kk / 64, r = (kk % 64) / 4, t = kk % 4), one u32
+// of four consecutive kk per plane.  Table entries or generator powers outside [0, n) / [1, 2n),
+// which the reference would use as out-of-range indices, are skipped.  Groups of 16 consecutive
+// columns (one tile row of the output) run on one XCD, so the strided reads of bold_t share L2
+// lines and the 16 u32 writes of a tile row meet in one L2.  LDS: K u64.
+__global__ __launch_bounds__(256) void pack_build_kernel(const uint64_t *__restrict__ bt, const uint64_t *__restrict__ bb,
+                                                         size_t out_stride, const uint32_t *__restrict__ tables,
+                                                         const uint32_t *__restrict__ gen_pows, int num_iter, int tg,
+                                                         int n, int cols, uint64_t q, int8_t *__restrict__ ub) {
+    extern __shared__ unsigned long long colk[];  // K = tg * n
+    const int L = blockIdx.x;
+    const int oz = cols % 128 == 0 ? ((L >> 3) & 15) + 16 * ((L & 7) + 8 * (L >> 7)) : L;
+    const int o = oz / n, z = oz % n, K = tg * n, KC = K / 64, CT = cols / 16;
+    for (int r = threadIdx.x; r < K; r += blockDim.x) colk[r] = 0;
+    __syncthreads();
+    const uint64_t *bto = bt + (size_t)o * out_stride + z, *bbo = bb + (size_t)o * out_stride + z;
+    for (int w = threadIdx.x; w < num_iter * tg; w += blockDim.x) {
+        const int i = w / tg, k = w % tg;
+        const uint32_t g = gen_pows[i];
+        if (g < 1 || g >= 2u * (uint32_t)n) continue;
+        const uint32_t zp = tables[(size_t)((g - 1) / 2) * n + z], zb = tables[(size_t)((2 * n - g - 1) / 2) * n + z];
+        // (i tg + k) n + z = w n + z
+        if (zp < (uint32_t)n) atomicAdd(&colk[k * n + zp], (unsigned long long)bto[(size_t)w * n]);
+        if (zb < (uint32_t)n) atomicAdd(&colk[k * n + zb], (unsigned long long)bbo[(size_t)w * n]);
+    }
+    __syncthreads();
+    const int ct = oz / 16, nn = oz % 16;
+    for (int q4 = threadIdx.x; q4 < K / 4; q4 += blockDim.x) {
+        const int kk0 = 4 * q4, kc = kk0 / 64, r = (kk0 % 64) / 4;
+        uint32_t wj[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const uint64_t v = colk[kk0 + t] % q;
+#pragma unroll
+            for (int j = 0; j < 4; j++) wj[j] |= ((uint32_t)((v >> (8 * j)) & 0xff) ^ 0x80u) << (8 * t);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            *(uint32_t *)(ub + (((size_t)j * (CT / 2) + ct / 2) * KC + kc) * 2048 + (ct % 2) * 1024 + r * 64 + 4 * nn) =
+                wj[j];
+    }
+}
+
 __device__ __forceinline__ int sum_i8x4(uint32_t w) {
     return (int)(int8_t)(w & 0xff) + (int)(int8_t)((w >> 8) & 0xff) + (int)(int8_t)((w >> 16) & 0xff) +
            (int)(int8_t)(w >> 24);

@@ -120,8 +120,8 @@ class TiledDb:
 
 def build_packing_matrix(ub: np.ndarray, bold_t: Sequence[np.ndarray], bold_t_bar: Sequence[np.ndarray],
                          tables: np.ndarray, gen_pows: np.ndarray, num_iter: int, tg: int, n: int, qmod: int,
-                         threads: int) -> None:
-    """src/engine.rs:127-138."""
+                         threads: int, device: bool = False) -> None:
+    """src/engine.rs:127-138.  device=True builds it on the GPU (alto_pack_build, the same bytes)."""
     rho = len(bold_t)
     assert len(ub) == 4 * tg * n * rho * n and len(bold_t_bar) == rho
     assert len(tables) == n * n and len(gen_pows) >= num_iter
@@ -131,8 +131,15 @@ def build_packing_matrix(ub: np.ndarray, bold_t: Sequence[np.ndarray], bold_t_ba
     pb = (C.c_void_p * rho)(*[_ptr(s) for s in bold_t_bar])
     tables = np.ascontiguousarray(tables, dtype=np.uint32)
     gen_pows = np.ascontiguousarray(gen_pows, dtype=np.uint32)
-    _lib.load().sw_pack_build(_ptr(ub), C.addressof(p), C.addressof(pb), _ptr(tables), _ptr(gen_pows), num_iter, tg,
-                              n, rho, qmod, threads)
+    lib = _lib.load()
+    if device:
+        r = lib.alto_pack_build(_ptr(ub), C.addressof(p), C.addressof(pb), _ptr(tables), _ptr(gen_pows), num_iter, tg,
+                                n, rho, qmod, threads)
+        if r != 0:
+            raise RuntimeError(f"alto_pack_build failed ({_lib.last_error()})")
+        return
+    lib.sw_pack_build(_ptr(ub), C.addressof(p), C.addressof(pb), _ptr(tables), _ptr(gen_pows), num_iter, tg, n, rho,
+                      qmod, threads)
 
 
 class Engine:

@@ -51,6 +51,18 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
                    const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg,
                    size_t n, size_t rho, uint64_t qmod, int threads);
 
+/* No counterpart in the reference: sw_pack_build computed on the device.  Same arguments and
+ * the same bytes in `ub` (host memory, 4 * tg * n * rho * n bytes) as src/amx_kernels.cpp:311-341;
+ * the inputs are copied to the device, each output column is summed in LDS (u64 wrap-around, as
+ * the reference's column) and reduced mod qmod there, and U is copied back.  `threads` is unused.
+ * Table entries outside [0, n) and generator powers outside [1, 2n), which the reference would
+ * use as out-of-range indices, are skipped.  Returns 0, or -1 (alto_last_error says why) for a
+ * shape it does not take (tg * n a multiple of 64 of at most 20,480, rho * n a multiple of 32),
+ * null pointers, no device or a failed device call. */
+int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *const *bold_t_bar,
+                    const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg, size_t n, size_t rho,
+                    uint64_t qmod, int threads);
+
 /* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
  * Returns nullptr for the configurations the reference rejects (Q != 4294955009,
  * n == 0 or n > 4096 or n % 16, cols % n, rows % 64, cols % 32, threads <= 0,
