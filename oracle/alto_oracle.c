@@ -349,6 +349,48 @@ void orc_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
     }
 }
 
+/* The reference's per-client online packing, before it collapses into one matrix: the sum
+ * over the packing rotations of src/packing.rs:2526-2566 (full_packing_with_preprocessing_online),
+ * for output o and NTT slot z of one client,
+ *   sum_o[z] = sum_{i < num_iter} sum_{k < tg}  rot_i,k[z] * bold_t[o][(i tg + k) n + z]
+ *                                              + rotbar_i,k[z] * bold_t_bar[o][(i tg + k) n + z]  (mod Q)
+ * where rot_i,k and rotbar_i,k are key polynomial y_k (NTT slots, y[k n .. k n + n)) under the
+ * automorphisms tau_g and tau_{2n-g}, g = gen_pows[i] (generate_rotations_double,
+ * src/packing.rs:1794-1816): in the NTT domain an automorphism permutes slots,
+ * out[j] = poly[table_t[j]] with table_t = tables[(t - 1) / 2] (apply_automorph_ntt_raw,
+ * src/packing.rs:1112-1126).  The rotations are materialised first and then multiplied
+ * pointwise, term by term, in the order of the reference loop; every product is reduced mod Q.
+ * The z_body x bold_t_hat term of the same sum (src/packing.rs:2556-2566) is the engine's tail
+ * key product and is not part of this sum.  beta[o * n + z] receives sum_o[z]. */
+void orc_pack_online(uint64_t *beta, const uint32_t *y, const uint64_t *const *bold_t,
+                     const uint64_t *const *bold_t_bar, const uint32_t *tables, const uint32_t *gen_pows,
+                     size_t num_iter, size_t tg, size_t n, size_t rho) {
+    uint64_t *rot = (uint64_t *)malloc(sizeof(uint64_t) * tg * n);
+    uint64_t *rotbar = (uint64_t *)malloc(sizeof(uint64_t) * tg * n);
+    uint64_t *sum = (uint64_t *)malloc(sizeof(uint64_t) * rho * n);
+    memset(sum, 0, sizeof(uint64_t) * rho * n);
+    for (size_t i = 0; i < num_iter; i++) {
+        const size_t g = gen_pows[i];
+        const uint32_t *tab = tables + ((g - 1) / 2) * n, *tabbar = tables + ((2 * n - g - 1) / 2) * n;
+        for (size_t k = 0; k < tg; k++)
+            for (size_t j = 0; j < n; j++) {
+                rot[k * n + j] = y[k * n + tab[j]] % ORC_Q;
+                rotbar[k * n + j] = y[k * n + tabbar[j]] % ORC_Q;
+            }
+        for (size_t o = 0; o < rho; o++)
+            for (size_t k = 0; k < tg; k++)
+                for (size_t z = 0; z < n; z++) {
+                    const size_t w = (i * tg + k) * n + z;
+                    sum[o * n + z] = (sum[o * n + z] + orc_mulmod(rot[k * n + z], bold_t[o][w] % ORC_Q)) % ORC_Q;
+                    sum[o * n + z] = (sum[o * n + z] + orc_mulmod(rotbar[k * n + z], bold_t_bar[o][w] % ORC_Q)) % ORC_Q;
+                }
+    }
+    memcpy(beta, sum, sizeof(uint64_t) * rho * n);
+    free(rot);
+    free(rotbar);
+    free(sum);
+}
+
 /* Database tiling for the reference's AMX path (setup), src/amx_kernels.cpp:138-149 with
  * sw_dbt (:42): 16-column x 64-record tiles, row r of a tile = 64 record bytes of column
  * 16*cb + r, tiles of column-tile pairs interleaved per 64-record chunk. */

@@ -23,7 +23,12 @@
  *   orc_rescale_q1    exhaustive comparison with orc_rescale_i64 (the
  *                     modulus_switch.rs formula) over every v in [0, Q)
  *   orc_write_bits    the golden bit patterns of src/bits.rs:115-133
- *   orc_pack_build    compiled reference (sw_pack_build)
+ *   orc_pack_build    compiled reference (sw_pack_build); with orc_pack, against
+ *                     orc_pack_online (the per-client rotation loop of the reference's
+ *                     online packing, which never builds the matrix)
+ *   orc_pack_online   the automorphism tables' closed form (pinned against NTT slot
+ *                     evaluation) and a brute-force evaluation of each automorphism on
+ *                     the coefficient polynomial
  *   orc_answer        compiled reference, byte for byte, on seeded inputs
  *   orc_matmul_vec_packed  compiled reference (src/matmul.cpp) and brute force
  */
@@ -83,6 +88,9 @@ void orc_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
                     const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg,
                     size_t n, size_t rho);
 void orc_tile_db(uint8_t *dst, const uint8_t *db, size_t ld, size_t rows, size_t cols);
+void orc_pack_online(uint64_t *beta, const uint32_t *y, const uint64_t *const *bold_t,
+                     const uint64_t *const *bold_t_bar, const uint32_t *tables, const uint32_t *gen_pows,
+                     size_t num_iter, size_t tg, size_t n, size_t rho);
 
 void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t rows, size_t cols,
                            size_t bc);

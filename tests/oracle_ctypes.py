@@ -66,6 +66,7 @@ def oracle() -> C.CDLL:
         lib.orc_answer.argtypes = [C.POINTER(OrcEngine), P, P, SZ, P, P]
         lib.orc_pack_build.argtypes = [P, P, P, P, P, SZ, SZ, SZ, SZ]
         lib.orc_tile_db.argtypes = [P, P, SZ, SZ, SZ]
+        lib.orc_pack_online.argtypes = [P, P, P, P, P, P, SZ, SZ, SZ, SZ]
         lib.orc_ntt_naive.argtypes = [P, P, SZ, C.c_uint64]
         lib.orc_intt_naive.argtypes = [P, P, SZ, C.c_uint64]
         lib.orc_powmod.restype = C.c_uint64

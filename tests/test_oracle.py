@@ -137,6 +137,95 @@ def test_ntt_slot_order_matches_the_automorphism_tables():
         assert all(int(AX[j]) == int(X[table[j]]) for j in range(n))
 
 
+def _auto_tables(n):
+    logn = n.bit_length() - 1
+    br = [workload.bitrev(j, logn) for j in range(n)]
+    tables = np.array([[br[((t * (2 * br[j] + 1)) % (2 * n) - 1) // 2] for j in range(n)]
+                       for t in range(1, 2 * n, 2)], dtype=np.uint32).reshape(-1)
+    gen_pows = np.array([pow(5, i, 2 * n) for i in range(n)], dtype=np.uint32)
+    return tables, gen_pows
+
+
+def _online_pack_by_automorphisms(y, bt, bb, n, tg, rho, num_iter, gen_pows):
+    """The rotation sum of the reference's online packing (src/packing.rs:2526-2566), with each
+    rotation computed without the slot tables: back to coefficients (naive inverse NTT), the
+    negacyclic automorphism X -> X^t there, forward again (naive NTT)."""
+    lib = oc.oracle()
+    psi = workload.primitive_root_2n(n)
+
+    def ntt(c):
+        out = np.zeros(n, np.uint64)
+        lib.orc_ntt_naive(out.ctypes.data, _u64(c).ctypes.data, n, psi)
+        return [int(v) for v in out]
+
+    def intt(v):
+        out = np.zeros(n, np.uint64)
+        lib.orc_intt_naive(out.ctypes.data, _u64(v).ctypes.data, n, psi)
+        return [int(x) for x in out]
+
+    def automorph(c, t):
+        a = [0] * n
+        for i, v in enumerate(c):
+            e = (i * t) % (2 * n)
+            if e < n:
+                a[e] = (a[e] + v) % Q
+            else:
+                a[e - n] = (a[e - n] - v) % Q
+        return a
+
+    coeffs = [intt([int(v) for v in y[k * n:(k + 1) * n]]) for k in range(tg)]
+    total = [[0] * n for _ in range(rho)]
+    for i in range(num_iter):
+        g = int(gen_pows[i])
+        for k in range(tg):
+            rot = ntt(automorph(coeffs[k], g))
+            rotbar = ntt(automorph(coeffs[k], 2 * n - g))
+            for o in range(rho):
+                for z in range(n):
+                    w = (i * tg + k) * n + z
+                    total[o][z] = (total[o][z] + rot[z] * int(bt[o][w]) + rotbar[z] * int(bb[o][w])) % Q
+    return np.array([v for o in range(rho) for v in total[o]], dtype=np.uint64)
+
+
+@pytest.mark.parametrize("n,tg,rho", [(16, 8, 2), (32, 4, 2)])  # tg n a multiple of 128, as the engine needs
+def test_online_packing_collapses_into_the_packing_matrix(n, tg, rho):
+    # the per-client online loop of the reference (rotate the keys by every automorphism of
+    # the packing, multiply by bold_t / bold_t_bar, add up) equals one product y . U with the
+    # matrix U that sw_pack_build assembles from the same bold_t / bold_t_bar: the identity the
+    # batched engine rests on (packing is linear in y and automorphisms are slot permutations)
+    lib = oc.oracle()
+    num_iter = n // 2 - 1
+    tables, gen_pows = _auto_tables(n)
+    kp = tg * n
+    bt = [synthetic.uniform_below_q(300 + o, num_iter * tg * n).astype(np.uint64) for o in range(rho)]
+    bb = [synthetic.uniform_below_q(400 + o, num_iter * tg * n).astype(np.uint64) for o in range(rho)]
+    pt = (C.c_void_p * rho)(*[x.ctypes.data for x in bt])
+    pb = (C.c_void_p * rho)(*[x.ctypes.data for x in bb])
+    for seed in (1, 2):
+        y = synthetic.uniform_below_q(500 + seed, kp)
+        online = np.zeros(rho * n, np.uint64)
+        lib.orc_pack_online(online.ctypes.data, y.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data,
+                            gen_pows.ctypes.data, num_iter, tg, n, rho)
+        # the rotations by coefficient-space automorphisms give the same sums (pins the slot tables'
+        # use in the restatement)
+        assert np.array_equal(online, _online_pack_by_automorphisms(y, bt, bb, n, tg, rho, num_iter, gen_pows))
+        ub = np.zeros(4 * kp * rho * n, np.int8)
+        lib.orc_pack_build(ub.ctypes.data, C.addressof(pt), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
+                           num_iter, tg, n, rho)
+        beta = np.zeros(rho * n, np.uint64)
+        ys = (C.c_void_p * 1)(y.ctypes.data)
+        lib.orc_pack(beta.ctypes.data, ub.ctypes.data, C.addressof(ys), 1, kp, rho * n)
+        assert np.array_equal(beta, online)
+        # a planted mistake (bold_t and bold_t_bar swapped for one rotation) is seen
+        bt2 = [x.copy() for x in bt]
+        bt2[0][:tg * n] = bb[0][:tg * n]
+        pt2 = (C.c_void_p * rho)(*[x.ctypes.data for x in bt2])
+        wrong = np.zeros(rho * n, np.uint64)
+        lib.orc_pack_online(wrong.ctypes.data, y.ctypes.data, C.addressof(pt2), C.addressof(pb), tables.ctypes.data,
+                            gen_pows.ctypes.data, num_iter, tg, n, rho)
+        assert not np.array_equal(wrong, beta)
+
+
 def test_intt_rejects_non_power_of_two():
     lib = oc.oracle()
     a = np.zeros(48, np.uint64)
