@@ -54,6 +54,42 @@ def measured_traffic(kernel: str, records: int, record_bytes: int, batch: int):
     return None
 
 
+def gpu_numa_cores() -> tuple[int, str]:
+    """Physical cores of the GPU's NUMA node that this process may run on (one thread per core:
+    the first logical CPU of each core in the node's list), and a description.  Falls back to
+    every CPU the process may use when the node is not known."""
+    allowed = sorted(os.sched_getaffinity(0))
+    node = None
+    try:
+        for dev in Path("/sys/class/drm").glob("card*/device"):
+            if (dev / "numa_node").exists() and (dev / "vendor").read_text().strip() == "0x1002":
+                node = int((dev / "numa_node").read_text().strip())
+                break
+    except OSError:
+        node = None
+    cpus = allowed
+    where = f"{len(allowed)} CPUs this process may use"
+    if node is not None and node >= 0:
+        try:
+            spec = Path(f"/sys/devices/system/node/node{node}/cpulist").read_text().strip()
+            in_node = set()
+            for part in spec.split(","):
+                a, _, b = part.partition("-")
+                in_node.update(range(int(a), int(b or a) + 1))
+            cpus = [c for c in allowed if c in in_node] or allowed
+            where = f"NUMA node {node} of the GPU, {len(cpus)} of its CPUs open to this process"
+        except OSError:
+            pass
+    cores = set()
+    for c in cpus:
+        try:
+            sib = Path(f"/sys/devices/system/cpu/cpu{c}/topology/thread_siblings_list").read_text().strip()
+            cores.add(sib.split(",")[0].split("-")[0])
+        except OSError:
+            cores.add(str(c))
+    return max(1, len(cores)), where + f", {len(cores)} physical cores"
+
+
 def cpu_baseline(inp, reqs, threads: int, budget_s: float) -> dict:
     """The reference engine compiled from its own source (oracle/_ref) on the host cores,
     method VNNI (AVX-512) or AMX, timed like `altopir bench` (src/bin/altopir.rs:741-826):
@@ -101,6 +137,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-numa", dest="cpu_numa", action="store_false",
+                    help="skip the second CPU baseline run on every physical core of the GPU's NUMA node")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--also-batch", type=int, nargs="*", default=[1],
                     help="further batch sizes measured on the same engine and reported under 'also'")
@@ -306,6 +344,22 @@ def main() -> int:
         plain = {"qps": round(B / dt, 2), "ms_per_step": round(dt * 1e3, 4), "steps": args.plain_steps,
                  "form": "alto_answer, host frames in pageable memory, responses copied back"}
         log(f"[bench] plain host form B={B}: {dt * 1e3:.3f} ms/batch, {B / dt:.1f} QPS")
+        # the same with frames and responses in pinned host memory (the copies go straight to the
+        # DMA engines, as the served form's do), so that batching is compared with batching
+        pin_in = torch.empty((B, L), dtype=torch.uint8, pin_memory=True)
+        pin_out = torch.empty((B, rl), dtype=torch.uint8, pin_memory=True)
+        pin_in.copy_(torch.from_numpy(np.stack(frames)))
+        pframes = [pin_in[c].numpy() for c in range(B)]
+        pout = pin_out.numpy()
+        eng.answer(pframes, pout, hst)
+        assert int((hst != 0).sum()) == 0 and np.array_equal(pout, hout)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            eng.answer(pframes, pout, hst)
+        dtp = (time.perf_counter() - t0) / args.plain_steps
+        plain["pinned"] = {"qps": round(B / dtp, 2), "ms_per_step": round(dtp * 1e3, 4),
+                           "form": "alto_answer, host frames and responses in pinned memory"}
+        log(f"[bench] plain host form, pinned frames B={B}: {dtp * 1e3:.3f} ms/batch, {B / dtp:.1f} QPS")
 
     served = None
     if args.serve_batches:
@@ -349,6 +403,12 @@ def main() -> int:
             if inp.db_col is None:
                 inp.db_col = synthetic.uniform_bytes(0, R * C)  # the same bytes, on the host
             cpu = cpu_baseline(inp, reqs, args.cpu_threads, args.cpu_budget)
+            if args.cpu_numa:
+                ncores, where = gpu_numa_cores()
+                numa = cpu_baseline(inp, reqs, ncores, args.cpu_budget)
+                numa["where"] = where
+                cpu["numa_node"] = numa
+                log(f"[bench] cpu baseline on the GPU's NUMA node: {numa}")
         else:
             # bounded sample: the first n columns (one output part) of the same database, the
             # same requests; scan, packing and tail all scale linearly in the columns
