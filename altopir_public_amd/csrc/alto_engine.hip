@@ -61,11 +61,33 @@ int choose_splits(int KC, long long tiles, int target, int min_steps) {
     return gz;
 }
 
+// Split count for shapes that hold one workgroup per CU: from ceil(KC / kFoldSteps) up to
+// four times that (and at least min_steps per split), the count that finishes in the least
+// time when the workgroups run in rounds of 256 and each costs its k-steps plus a start-up
+// of 16 steps; the smallest count within 2% of the best.  A grid of 1.6 rounds otherwise
+// leaves 40% of the CUs idle in its second round.
+int balanced_splits(int KC, long long tiles, int min_steps) {
+    const int gz0 = ceil_div(KC, kFoldSteps);
+    double cost[64];
+    int last = gz0;
+    double best = 1e300;
+    for (int gz = gz0; gz <= 4 * gz0 && gz - gz0 < 64 && (gz == gz0 || KC / gz >= min_steps); ++gz) {
+        const long long rounds = (tiles * gz + 255) / 256;
+        cost[gz - gz0] = (double)rounds * ((double)KC / gz + 16.0);
+        if (cost[gz - gz0] < best) best = cost[gz - gz0];
+        last = gz;
+    }
+    for (int gz = gz0; gz <= last; ++gz)
+        if (cost[gz - gz0] <= 1.02 * best) return gz;
+    return gz0;
+}
+
 // Workgroup grid of one contraction (sets g.gx, g.gy, g.gz; returns the workgroup count):
 // streaming shapes split the record range until about WGS workgroups (1024 unless a variant
 // says otherwise: measured best or level at 1-64 GiB, 7% faster than 2048 at 8 GiB), ring
-// shapes until 256 (one per CU); the packing contraction is never split (KP fits one i32
-// range).
+// shapes by balanced_splits (at 8 GiB the scan of 37 queries on 12 x 20 tiles takes 7 splits,
+// 721 workgroups in 2.8 rounds, instead of 4 in 1.6 rounds: 4-9% faster on two boxes, and
+// 3-7% at 33 and 48 queries); the packing contraction is never split (KP fits one i32 range).
 template <int MB, int NBT, int WGS = 1024>
 int plan_stream(GemmArgs &g, bool pack) {
     g.gx = ceil_div(g.NT, 4 * NBT);
@@ -77,7 +99,7 @@ template <int TM, int TN, int WM, int WN>
 int plan_ring(GemmArgs &g, bool pack) {
     g.gx = ceil_div(g.NT, WN * TN);
     g.gy = ceil_div(g.MT, WM * TM);
-    g.gz = pack ? 1 : choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
+    g.gz = pack ? 1 : balanced_splits(g.KC, (long long)g.gx * g.gy, 64);
     return 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
 }
 
@@ -94,11 +116,7 @@ void launch_stream(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st)
 
 template <int TM, int TN, int WM, int WN, bool PACK>
 void launch_lds(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
-    g.gx = ceil_div(g.NT, WN * TN);
-    g.gy = ceil_div(g.MT, WM * TM);
-    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
-    else g.gz = 1;
-    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((lds_kernel<TM, TN, WM, WN, PACK, true>), dim3(grid), dim3(256), 0, st, (const i32x4 *)a,
                            (const i32x4 *)b, g);
@@ -144,11 +162,7 @@ void prepare_dma2() {
 template <int TM, int TN, int WM, int WN, int S, bool PACK>
 void launch_dma2(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
     constexpr size_t lds = (size_t)S * (WM * TM + WN * TN) * 1024;
-    g.gx = ceil_div(g.NT, WN * TN);
-    g.gy = ceil_div(g.MT, WM * TM);
-    if (!PACK) g.gz = choose_splits(g.KC, (long long)g.gx * g.gy, 256, 64);
-    else g.gz = 1;
-    const int grid = 8 * ceil_div(g.gx, 8) * g.gy * g.gz;
+    const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((dma2_kernel<TM, TN, WM, WN, S, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds, st,
                            (const i32x4 *)a, (const i32x4 *)b, g);

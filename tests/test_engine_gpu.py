@@ -211,13 +211,14 @@ def test_every_kernel_variant_gives_the_same_bytes(hip_device, B):
     eng.close()
 
 
-@pytest.mark.parametrize("B", [5, 72])
-def test_every_kernel_variant_on_an_odd_number_of_k_steps(hip_device, B):
+@pytest.mark.parametrize("steps, B", [(37, 5), (37, 72), (453, 37)])
+def test_every_kernel_variant_on_an_odd_number_of_k_steps(hip_device, steps, B):
     # 2,368 records = 37 k-steps of 64 (a prime): record-range splits and the unrolled k-loops
-    # of the contraction kernels end on partial groups of steps
+    # of the contraction kernels end on partial groups of steps; 453 k-steps: the ring shapes
+    # split the record range 4 ways, into ranges of 113 and 114 steps
     from altopir_public_amd import _lib, autotune
 
-    inp = workload.make_inputs(64 * 37, 4096, n=2048, tg=4, seed=37, prefix="random")
+    inp = workload.make_inputs(64 * steps, 4096, n=2048, tg=4, seed=steps, prefix="random")
     reqs = workload.make_requests(B, inp.rows, inp.kp, seed=600 + B)
     rl = inp.response_len
     out_o = np.zeros((B, rl), np.uint8)
