@@ -173,7 +173,7 @@ void launch_dma2(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
 
 template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
 void prepare_dmab() {
-    constexpr size_t lds = (size_t)(D + 2) * WM * TM * 1024;
+    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false>::kLds;
     (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false>,
@@ -182,7 +182,7 @@ void prepare_dmab() {
 
 template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
 void launch_dmab(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
-    constexpr size_t lds = (size_t)(D + 2) * WM * TM * 1024;
+    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false>::kLds;
     const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
         hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds,
@@ -274,6 +274,12 @@ const Variant kVariants[] = {
      prepare_dmab<8, 4, 2, 4, 2, 8, false>, prepare_dmab<8, 4, 2, 4, 2, 8, true>},
     {"dmab8_m8_n4_u16", launch_dmab<8, 4, 2, 4, 2, 16, false>, launch_dmab<8, 4, 2, 4, 2, 16, true>,
      prepare_dmab<8, 4, 2, 4, 2, 16, false>, prepare_dmab<8, 4, 2, 4, 2, 16, true>},
+    // the same with 12 x 16 and 8 x 32-tile workgroups (the 8 x 32 one with one wave per
+    // column of four B tiles), for 17-48 queries
+    {"dmab8_m6_n4_u16", launch_dmab<6, 4, 2, 4, 2, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
+     prepare_dmab<6, 4, 2, 4, 2, 16, false>, prepare_dmab<6, 4, 2, 4, 2, 16, true>},
+    {"dmab8_w18_m8_n4_u16", launch_dmab<8, 4, 1, 8, 2, 16, false>, launch_dmab<8, 4, 1, 8, 2, 16, true>,
+     prepare_dmab<8, 4, 1, 8, 2, 16, false>, prepare_dmab<8, 4, 1, 8, 2, 16, true>},
     {"dma2_8w_m4_n8_s4", launch_dma2<4, 8, 4, 2, 4, false>, launch_dma2<4, 8, 4, 2, 4, true>,
      prepare_dma2<4, 8, 4, 2, 4, false>, prepare_dma2<4, 8, 4, 2, 4, true>},
     {"dma2_8w_m4_n8_s3", launch_dma2<4, 8, 4, 2, 3, false>, launch_dma2<4, 8, 4, 2, 3, true>,
@@ -375,18 +381,20 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 // shape, whose loads issue in a better order: 2% faster at 1-4 GiB, even at 64 GiB); the
 // scan of 9-32 queries on 8 x 32-tile workgroups (each streamed query row meets twice the
 // database columns of the square tile) and its packing on 8 x 16 tiles; the scan of 33-48
-// queries on 12-row tiles; above that both contractions on the 16 x 16-tile workgroups that
-// stream the B blocks into registers, 16 k-steps per loop iteration (at batch 128: scan
-// 0.432 against 0.492 ms for the best all-LDS ring at 1 GiB, 30.3 against 30.6 ms at
-// 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB batch 2048 59.2 against 60.1 ms).
+// queries on 12-row tiles and its packing on 12 x 16 tiles that stream the B blocks into
+// registers (0.231 against 0.253 ms on 16 x 16 at 8 GiB and 37 queries); above that both
+// contractions on the 16 x 16-tile workgroups that stream the B blocks into registers, 16
+// k-steps per loop iteration (at batch 128: scan 0.432 against 0.492 ms for the best all-LDS
+// ring at 1 GiB, 30.3 against 30.6 ms at 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB
+// batch 2048 59.2 against 60.1 ms).
 int variant_index(const char *name);
 int default_variant(int which, int MT) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
                      rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
-                     regb = variant_index("dmab8_m8_n4_u16");
+                     regb = variant_index("dmab8_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16");
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
-    if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : regb;
+    if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
     if (MT <= 8) return rows8;
     if (MT <= 12) return rows12;
     return regb;

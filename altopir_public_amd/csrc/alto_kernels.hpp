@@ -923,7 +923,10 @@ __device__ __align__(16) i32x4 kZeroBlock[64];
 // by the WN waves of a row), while every wave loads its own TN B blocks into a register ring
 // D k-steps ahead (the WM waves of a column load the same blocks; the second request hits the
 // CU's L1).  Per k-step a CU moves BM KiB into LDS and reads WM * WN * TM KiB out of it:
-// with 16 x 16 tiles 80 KiB of LDS traffic against 128 KiB for the all-LDS ring.
+// with 16 x 16 tiles 80 KiB of LDS traffic against 128 KiB for the all-LDS ring.  Each wave
+// copies the same number of A blocks per step (the wait counts below assume it), so a ring
+// slot holds BM rounded up to a multiple of the eight waves; the rows past BM are loaded
+// from clamped addresses and never read.
 //
 // The A fragments of step k + 1 are read into the registers of step k's fragments right
 // after the MFMAs that use them, so the barrier that publishes step k + 1 comes before step
@@ -942,16 +945,16 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
     constexpr int NW = WM * WN;
     static_assert(NW == 8, "two waves per SIMD");
     constexpr int BM = WM * TM;
-    static_assert(BM % NW == 0, "A blocks split evenly over the waves");
     static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
     static_assert(D >= 2 && D <= 3, "prefetch distance");
     static_assert(UNR % D == 0, "whole register rings per loop iteration");
     constexpr int S = D + 2;
-    constexpr int PA = BM / NW;         // A blocks each wave copies per step
+    constexpr int PA = (BM + NW - 1) / NW;  // A blocks each wave copies per step
+    constexpr int BMP = PA * NW;            // A blocks per ring slot
     constexpr int PAIR = PA + TN;       // vector-memory instructions per step and wave
     static_assert(D * PAIR < 64, "vmcnt range");
     constexpr int BS = b_kstride<PACK>();
-    extern __shared__ i32x4 ring[];     // [S][BM][64]
+    extern __shared__ i32x4 ring[];     // [S][BMP][64]
     const TileMap tm = map_tile_at(L, g.gx, g.gy, g.gz);
     if (!tm.valid) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -968,7 +971,7 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
     for (int n = 0; n < TN; n++) bsrc[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
     const i32x4 *zero = kZeroBlock + lane;
     auto issue_a = [&](int t) {  // A(t) into its slot (t past the range: the last block)
-        i32x4 *dst = ring + (size_t)(((t - kb) % S) * BM) * 64;
+        i32x4 *dst = ring + (size_t)(((t - kb) % S) * BMP) * 64;
         const size_t kk = (size_t)min(t, ke - 1);
 #pragma unroll
         for (int p = 0; p < PA; p++)
@@ -1016,7 +1019,7 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
                 const int k = k0 + jj;
                 wait_vmcnt<(D - 1) * PAIR>();  // P(k): A(k + 1) and B(k)
                 stage_barrier();               // A(k + 1) visible; slot of step k - 1 free
-                const i32x4 *nxt = ring + (size_t)(((k + 1 - kb) % S) * BM + wm * TM) * 64 + lane;
+                const i32x4 *nxt = ring + (size_t)(((k + 1 - kb) % S) * BMP + wm * TM) * 64 + lane;
                 // kept in this order (scheduling barriers): hoisting a fragment read or the B
                 // loads above the MFMAs that free their registers would double the live
                 // operands
@@ -1083,7 +1086,7 @@ struct DmaShape {
 template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
 struct DmabShape {
     static constexpr int kThreads = 64 * WM * WN;
-    static constexpr size_t kLds = (size_t)(D + 2) * WM * TM * kBlockBytes;
+    static constexpr size_t kLds = (size_t)(D + 2) * ((WM * TM + 7) / 8 * 8) * kBlockBytes;
     static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
         dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a, b, g, L);
     }
