@@ -423,7 +423,10 @@ __global__ void batch_dims_kernel(int *__restrict__ dims, int B, int MT) {
 
 // Maps a 1-D workgroup id onto (x, y, z) tiles so that the workgroups of one column tile
 // x (all y, same z) run back to back on the same XCD (dispatch round-robins workgroups
-// over the 8 XCDs): they then share the database slice in that XCD's L2.
+// over the 8 XCDs): they then share the database slice in that XCD's L2.  With more than
+// eight row blocks (and whole groups of them and of columns) the workgroups an XCD runs
+// together cover 8 row blocks x 4 column tiles instead of every row block of one column:
+// 12 operand streams per k-step for 32 workgroups instead of gy + 1.
 struct TileMap {
     int x, y, z;
     bool valid;
@@ -433,10 +436,23 @@ __device__ __forceinline__ TileMap map_tile_at(int L, int gx, int gy, int gz) {
     int t = L >> 3;
     const int gxg = (gx + 7) >> 3;
     TileMap m;
-    m.y = t % gy;
-    t /= gy;
-    m.x = (t % gxg) * 8 + xcd;
-    m.z = t / gxg;
+    if (gy > 8 && gy % 8 == 0 && gxg % 4 == 0) {
+        const int yy = t & 7;
+        t >>= 3;
+        const int xx = t & 3;
+        t >>= 2;
+        const int ry = t % (gy >> 3);
+        t /= gy >> 3;
+        const int cx = t % (gxg >> 2);
+        m.y = ry * 8 + yy;
+        m.x = ((cx << 2) + xx) * 8 + xcd;
+        m.z = t / (gxg >> 2);
+    } else {
+        m.y = t % gy;
+        t /= gy;
+        m.x = (t % gxg) * 8 + xcd;
+        m.z = t / gxg;
+    }
     m.valid = m.x < gx && m.z < gz;
     return m;
 }

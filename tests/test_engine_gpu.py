@@ -173,6 +173,32 @@ def test_empty_batch_and_null_frames(hip_device, small_inputs):
     eng.close()
 
 
+def test_a_batch_of_many_row_blocks_matches_smaller_batches_and_the_oracle(hip_device):
+    # 1,000 queries = 250 A tiles: 16 row blocks of the 16-row shapes, whose workgroups the tile
+    # map groups as 8 row blocks x 4 column tiles per XCD (8,192 columns = 32 column tiles of
+    # 256).  The same frames answered in batches of 250 (4 row blocks, one column tile's row
+    # blocks together) give the same bytes, and four of them the oracle's.
+    inp = workload.make_inputs(2048, 8192, n=2048, tg=4, seed=77, prefix="random")
+    B, rl = 1000, inp.response_len
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=78)
+    eng = Engine.from_inputs(inp)
+    out = np.zeros((B, rl), np.uint8)
+    st = np.full(B, -1, np.int32)
+    assert eng.answer(reqs, out, st)[0] == B
+    part = np.zeros_like(out)
+    for c0 in range(0, B, 250):
+        stp = np.full(250, -1, np.int32)
+        assert eng.answer(reqs[c0:c0 + 250], part[c0:c0 + 250], stp)[0] == 250
+    eng.close()
+    bad = [c for c in range(B) if not np.array_equal(out[c], part[c])]
+    assert not bad, f"responses {bad[:8]} differ between the batch of 1000 and batches of 250"
+    pick = [0, 499, 613, 999]
+    out_o = np.zeros((len(pick), rl), np.uint8)
+    assert oc.oracle_answer(inp, [reqs[i] for i in pick], out_o, np.full(len(pick), -1, np.int32)) == len(pick)
+    for k, i in enumerate(pick):
+        assert np.array_equal(out[i], out_o[k]), f"response {i} differs from the oracle"
+
+
 def test_repeated_batches_of_different_sizes_reuse_the_engine(hip_device, small_inputs):
     inp = small_inputs
     eng = Engine.from_inputs(inp)
