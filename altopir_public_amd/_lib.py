@@ -24,6 +24,8 @@ SYMBOLS = (
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
     "alto_server_get_stats", "alto_server_error", "alto_server_free", "alto_server_replay", "alto_server_set_graphs",
     "alto_server_set_upload", "alto_server_upload_mode",
+    # TCP front end (bound in net.py)
+    "alto_net_new", "alto_net_port", "alto_net_wait", "alto_net_get_stats", "alto_net_free",
 )
 
 _lib = None

@@ -3,7 +3,8 @@
     python -m altopir_public_amd.build        # -> altopir_public_amd/libaltopir_hip.so
 
 The library exports the C ABI of include/altopir_engine.h.  It is built with hipcc
-directly (no cmake/ninja): one translation unit holds the kernels and the host side.
+directly (no cmake/ninja): the kernels and the engine's host side, the serving loop and its TCP
+front end (host code only).
 """
 from __future__ import annotations
 
@@ -39,7 +40,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / "alto_engine.hip"), str(CSRC / "alto_serve.hip")]
+    srcs = [str(CSRC / "alto_engine.hip"), str(CSRC / "alto_serve.hip"), str(CSRC / "alto_net.cpp")]
     tmp = LIB.with_suffix(".so.tmp")
     # the serving loop issues its uploads to the DMA engines through the HSA runtime
     rocm_lib = Path(hipcc()).resolve().parent.parent / "lib"

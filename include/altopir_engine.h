@@ -316,6 +316,35 @@ int alto_server_set_upload(void *srv, int mode);
 /* The upload mode in force for database db (0-3 as above), -1 on bad arguments. */
 int alto_server_upload_mode(void *srv, int db);
 
+/* TCP front end of the serving loop.  No counterpart in the reference's C ABI: it replaces
+ * the reference server's connection readers and writers (src/bin/altopir.rs:403-442, :383-400)
+ * and speaks their wire protocol (src/bin/altopir.rs:282-308), little-endian:
+ *   request [u32 len][u64 id][len bytes]; reply [u32 len][u64 id][i32 status][len bytes]
+ *   (status 0 with the response, otherwise len = 0); stop frame len = 0, id = 2^64 - 1.
+ * Database i of `srv` listens on host:ports[i] (IPv4; port 0 takes a free port, see
+ * alto_net_port); `engines` is the array given to alto_server_new (the request length of each
+ * database comes from its engine).  A frame longer than the request length is discarded
+ * without buffering and answered with status 1; a frame of another wrong length gets status 1
+ * and a full slot pool status 9 (the reference's busy status) from alto_server_submit; the
+ * device's validation gives 2 and 3.  Replies of one connection go out in completion order.
+ * The front end polls the server's completions itself: the caller must not call
+ * alto_server_poll or alto_server_replay on that server while it runs.  Returns null on bad
+ * arguments or when a port cannot be bound. */
+typedef struct {
+    long long connections, frames, answered, rejected, busy, discarded, stop_frames, bytes_in, bytes_out;
+} alto_net_stats;
+void *alto_net_new(void *srv, void *const *engines, int n_engines, const char *host, const int *ports);
+/* The port database db listens on, -1 on bad arguments. */
+int alto_net_port(const void *net, int db);
+/* Waits up to timeout_ms (< 0: without limit) for a stop frame on any connection; returns 1
+ * once one has arrived, 0 on time-out, -1 on bad arguments. */
+int alto_net_wait(void *net, int timeout_ms);
+int alto_net_get_stats(const void *net, alto_net_stats *out);
+/* Stops listening, closes every connection (replies not yet written are dropped and their
+ * response slots released) and joins the front end's threads.  Free it before its server;
+ * requests in flight at that moment complete into the server's completion queue. */
+void alto_net_free(void *net);
+
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */
 size_t alto_engine_device_bytes(const void *e);
