@@ -25,7 +25,7 @@ SYMBOLS = (
     "alto_server_get_stats", "alto_server_error", "alto_server_free", "alto_server_replay", "alto_server_set_graphs",
     "alto_server_set_upload", "alto_server_upload_mode",
     # TCP front end (bound in net.py)
-    "alto_net_new", "alto_net_port", "alto_net_wait", "alto_net_get_stats", "alto_net_free",
+    "alto_net_new", "alto_net_port", "alto_net_wait", "alto_net_get_stats", "alto_net_free", "alto_net_load",
 )
 
 _lib = None

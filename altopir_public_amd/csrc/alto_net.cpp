@@ -21,9 +21,12 @@
 #include <sys/uio.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cerrno>
 #include <chrono>
+#include <climits>
+#include <cmath>
 #include <condition_variable>
 #include <cstdint>
 #include <cstdio>
@@ -425,3 +428,166 @@ void alto_net_free(void *p) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ native load client --
+
+namespace {
+
+// xoshiro256** seeded through splitmix64 (the schedule's own generator)
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        for (auto &w : s) {
+            seed += 0x9e3779b97f4a7c15ull;
+            uint64_t z = seed;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            w = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
+};
+
+int64_t mono_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+int dial(const char *host, int port, int timeout_s) {
+    in_addr addr{};
+    if (inet_pton(AF_INET, host, &addr) != 1) return -1;
+    const int fd = socket(AF_INET, SOCK_STREAM | SOCK_CLOEXEC, 0);
+    if (fd < 0) return -1;
+    sockaddr_in sa{};
+    sa.sin_family = AF_INET;
+    sa.sin_port = htons((uint16_t)port);
+    sa.sin_addr = addr;
+    if (connect(fd, (sockaddr *)&sa, sizeof sa) != 0) {
+        close(fd);
+        return -1;
+    }
+    const int one = 1;
+    setsockopt(fd, IPPROTO_TCP, TCP_NODELAY, &one, sizeof one);
+    timeval tv{timeout_s, 0};
+    setsockopt(fd, SOL_SOCKET, SO_RCVTIMEO, &tv, sizeof tv);
+    return fd;
+}
+
+}  // namespace
+
+extern "C" int alto_net_load(const char *host, int port, const uint8_t *const *frames, const size_t *lens, int nframes,
+                             const int *expect_status, const uint8_t *const *expect, size_t expect_len, double rate,
+                             double seconds, int conns, uint64_t seed, int drain_ms, alto_net_load_result *out) {
+    if (!host || !frames || !lens || nframes <= 0 || !(rate > 0) || !(seconds > 0) || conns <= 0 || !out) return -1;
+    // the schedule (src/bin/altopir.rs:612-623): exponential gaps at `rate`, frames uniform
+    Rng rng(seed);
+    std::vector<int64_t> due;
+    double t = 0;
+    for (;;) {
+        t += -std::log(1.0 - rng.uniform()) / rate;
+        if (t >= seconds) break;
+        due.push_back((int64_t)(t * 1e9));
+    }
+    const size_t n = due.size();
+    std::vector<int> idx(n);
+    for (auto &v : idx) v = (int)(rng.next() % (uint64_t)nframes);
+    std::vector<int> fds(conns, -1);
+    for (int c = 0; c < conns; c++) {
+        fds[c] = dial(host, port, (int)seconds + drain_ms / 1000 + 5);
+        if (fds[c] < 0) {
+            for (int fd : fds)
+                if (fd >= 0) close(fd);
+            return -1;
+        }
+    }
+    std::vector<int> status(n, INT32_MIN);
+    std::vector<int64_t> t_recv(n, 0);
+    std::vector<uint8_t> mismatch(n, 0);
+    const int64_t start = mono_ns() + 200000000;  // 0.2 s to let every thread start
+    std::vector<std::thread> th;
+    for (int c = 0; c < conns; c++) {
+        th.emplace_back([&, c] {  // sender (src/bin/altopir.rs:665-684)
+            for (size_t i = (size_t)c; i < n; i += (size_t)conns) {
+                const int64_t at = start + due[i];
+                for (;;) {
+                    const int64_t now = mono_ns();
+                    if (now >= at) break;
+                    if (at - now > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(at - now - 150000));
+                }
+                uint8_t h[12];
+                const uint32_t len = (uint32_t)lens[idx[i]];
+                const uint64_t id = i;
+                memcpy(h, &len, 4);
+                memcpy(h + 4, &id, 8);
+                iovec iov[2] = {{h, 12}, {(void *)frames[idx[i]], len}};
+                if (!write_all(fds[c], iov, len ? 2 : 1)) return;
+            }
+            std::this_thread::sleep_for(std::chrono::milliseconds(drain_ms));
+            shutdown(fds[c], SHUT_WR);
+        });
+        th.emplace_back([&, c] {  // receiver (src/bin/altopir.rs:650-664)
+            const size_t mine = n > (size_t)c ? (n - (size_t)c + conns - 1) / (size_t)conns : 0;
+            std::vector<uint8_t> body(expect_len > 0 ? expect_len : 1);
+            for (size_t k = 0; k < mine; k++) {
+                uint8_t h[16];
+                if (!read_full(fds[c], h, 16)) return;
+                uint32_t len;
+                uint64_t id;
+                int st;
+                memcpy(&len, h, 4);
+                memcpy(&id, h + 4, 8);
+                memcpy(&st, h + 12, 4);
+                if (len > body.size()) body.resize(len);
+                if (len && !read_full(fds[c], body.data(), len)) return;
+                if (id >= n) continue;
+                t_recv[id] = mono_ns();
+                status[id] = st;
+                if (st == 0 && expect && expect[idx[id]])
+                    mismatch[id] = (len != expect_len || memcmp(body.data(), expect[idx[id]], len) != 0) ? 2 : 1;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (int fd : fds) close(fd);
+    alto_net_load_result r{};
+    r.requests = (long long)n;
+    r.seconds = seconds;
+    std::vector<double> lat;
+    for (size_t i = 0; i < n; i++) {
+        if (status[i] == INT32_MIN) {
+            r.missing++;
+            continue;
+        }
+        if (status[i] == 0) {
+            r.answered++;
+            lat.push_back((double)(t_recv[i] - start - due[i]) * 1e-6);
+        } else if (status[i] == 9) {
+            r.busy++;
+        } else {
+            r.rejected++;
+        }
+        if (expect_status && status[i] != 9 && status[i] != expect_status[idx[i]]) r.wrong_status++;
+        if (mismatch[i]) r.checked++;
+        if (mismatch[i] == 2) r.bad++;
+    }
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, (size_t)(lat.size() * p))]; };
+    r.answered_qps = (double)r.answered / seconds;
+    r.p50_ms = pct(0.5);
+    r.p90_ms = pct(0.9);
+    r.p99_ms = pct(0.99);
+    r.max_ms = lat.empty() ? 0.0 : lat.back();
+    *out = r;
+    return 0;
+}

@@ -10,8 +10,10 @@ Frames (src/bin/altopir.rs:282-308), little-endian:
     python -m altopir_public_amd.net --gib 1 --rate 2000 --seconds 10 --conns 16
 
 serves one synthetic database (R = gib * 2^30 / C records of C uniform bytes, seed 0) on
-127.0.0.1, sends requests on a Poisson schedule over `--conns` connections from pre-generated
-frames (1% corrupted, one in 500 longer than a request), checks every reply's status against the
+127.0.0.1, sends requests on a Poisson schedule over `--conns` connections (native client
+threads, alto_net_load, unless --client python) from pre-generated
+frames (--frames valid ones, one corrupted per hundred of them and one longer than any request,
+chosen uniformly per request), checks every reply's status against the
 decoder rule and a sample of responses against direct answers of the engine, stops the server
 with a stop frame and prints one JSON line.
 """
@@ -40,10 +42,19 @@ class NetStats(C.Structure):
                                             "stop_frames", "bytes_in", "bytes_out")]
 
 
+class LoadResult(C.Structure):
+    _fields_ = [(f, C.c_longlong) for f in ("requests", "answered", "busy", "rejected", "missing", "wrong_status",
+                                            "checked", "bad")] + \
+               [(f, C.c_double) for f in ("seconds", "answered_qps", "p50_ms", "p90_ms", "p99_ms", "max_ms")]
+
+
 def _bind(lib):
     if getattr(lib, "_net_bound", False):
         return lib
     P = C.c_void_p
+    lib.alto_net_load.restype = C.c_int
+    lib.alto_net_load.argtypes = [C.c_char_p, C.c_int, P, P, C.c_int, P, P, C.c_size_t, C.c_double, C.c_double,
+                                  C.c_int, C.c_uint64, C.c_int, C.POINTER(LoadResult)]
     lib.alto_net_new.restype = P
     lib.alto_net_new.argtypes = [P, P, C.c_int, C.c_char_p, C.POINTER(C.c_int)]
     lib.alto_net_port.restype = C.c_int
@@ -211,6 +222,39 @@ def tcp_load(host: str, port: int, frames, rate: float, seconds: float, conns: i
             "wrong_status": wrong_status, "checked_responses": checked, "bad_responses": bad}
 
 
+def native_load(host: str, port: int, frames, rate: float, seconds: float, conns: int = 16, seed: int = 1,
+                drain_s: float = 10.0, expect_status=None, expect=None) -> dict:
+    """The same open-loop load from native sender/receiver threads (alto_net_load), so that the
+    client does not bound the rate; `expect` maps frame index -> response bytes."""
+    lib = _bind(_lib.load())
+    arrs = [np.frombuffer(bytes(f), np.uint8) for f in frames]
+    keep = [a if len(a) else np.zeros(1, np.uint8) for a in arrs]
+    fp = (C.c_void_p * len(arrs))(*[k.ctypes.data for k in keep])
+    ln = (C.c_size_t * len(arrs))(*[len(a) for a in arrs])
+    es = (C.c_int * len(arrs))(*expect_status) if expect_status is not None else None
+    ex, elen, hold = None, 0, []
+    if expect:
+        elen = len(next(iter(expect.values())))
+        ex = (C.c_void_p * len(arrs))()
+        for i, r in expect.items():
+            b = np.frombuffer(r, np.uint8)
+            assert len(b) == elen
+            hold.append(b)
+            ex[i] = b.ctypes.data
+    r = LoadResult()
+    rc = lib.alto_net_load(host.encode(), port, C.addressof(fp), C.addressof(ln), len(arrs),
+                           C.addressof(es) if es is not None else None, C.addressof(ex) if ex is not None else None,
+                           elen, rate, seconds, conns, seed, int(drain_s * 1000), C.byref(r))
+    if rc != 0:
+        raise RuntimeError("alto_net_load: bad arguments or no connection")
+    d = {f: getattr(r, f) for f, _ in LoadResult._fields_}
+    return {"requests": d["requests"], "answered": d["answered"], "busy": d["busy"], "rejected": d["rejected"],
+            "missing": d["missing"], "answered_qps": round(d["answered_qps"], 2), "conns": conns, "rate": rate,
+            "seconds": seconds, "latency_ms": {k: round(d[k + "_ms"], 3) for k in ("p50", "p90", "p99", "max")},
+            "wrong_status": d["wrong_status"], "checked_responses": d["checked"], "bad_responses": d["bad"],
+            "client": "native"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gib", type=float, default=1.0)
@@ -223,6 +267,8 @@ def main(argv=None) -> int:
     ap.add_argument("--slots", type=int, default=512, help="request slots (the reference's --queue)")
     ap.add_argument("--contexts", type=int, default=2)
     ap.add_argument("--check", type=int, default=16, help="frames whose responses are compared")
+    ap.add_argument("--client", choices=["native", "python"], default="native",
+                    help="load client: native threads (alto_net_load) or Python threads")
     a = ap.parse_args(argv)
 
     from . import workload
@@ -233,7 +279,8 @@ def main(argv=None) -> int:
     frames = frame_pool(inp, a.frames, seed=5)
     rng = np.random.default_rng(6)
     L = inp.request_len
-    # 1% corrupted (statuses 1-3 by the decoder rule), one in 500 longer than any request
+    # one corrupted frame per hundred valid ones (statuses 1-3 by the decoder rule), and one
+    # longer than any request
     extra = []
     for i in range(max(1, a.frames // 100)):
         extra.append(np.frombuffer(workload.corrupt(frames[i], rng, inp.kp), np.uint8))
@@ -250,7 +297,9 @@ def main(argv=None) -> int:
     net = NetServer(srv)
     port = net.port(0)
     print(f"[net] listening on 127.0.0.1:{port}", file=sys.stderr, flush=True)
-    r = tcp_load("127.0.0.1", port, allf, a.rate, a.seconds, conns=a.conns, expect_status=statuses, expect=expect)
+    run = native_load if a.client == "native" else tcp_load
+    r = run("127.0.0.1", port, allf, a.rate, a.seconds, conns=a.conns, drain_s=3.0, expect_status=statuses,
+            expect=expect)
     s = connect("127.0.0.1", port)
     send_stop(s)
     stopped = net.wait(5000)

@@ -345,6 +345,24 @@ int alto_net_get_stats(const void *net, alto_net_stats *out);
  * requests in flight at that moment complete into the server's completion queue. */
 void alto_net_free(void *net);
 
+/* Native open-loop load client for the front end above, after the reference's `load` command
+ * (src/bin/altopir.rs:594-739): requests are due at exponential gaps of mean 1/rate over
+ * `seconds` (generator seeded with `seed`), request i carries frame idx_i (uniform over the
+ * nframes frames) and id i, and is sent at its due time over connection i % conns, each
+ * connection with a sender and a receiver thread; senders half-close their connection
+ * drain_ms after their last request.  Latency runs from the due time to the reply.
+ * expect_status (per frame, may be null) and expect (per frame a response of expect_len bytes
+ * or null, array may be null) check the replies: wrong_status counts replies other than the
+ * expected status or 9, checked/bad the answered replies compared.  Returns 0, -1 on bad
+ * arguments or when a connection cannot be opened. */
+typedef struct {
+    long long requests, answered, busy, rejected, missing, wrong_status, checked, bad;
+    double seconds, answered_qps, p50_ms, p90_ms, p99_ms, max_ms;
+} alto_net_load_result;
+int alto_net_load(const char *host, int port, const uint8_t *const *frames, const size_t *lens, int nframes,
+                  const int *expect_status, const uint8_t *const *expect, size_t expect_len, double rate,
+                  double seconds, int conns, uint64_t seed, int drain_ms, alto_net_load_result *out);
+
 /* Number of HBM bytes the engine holds for its resident state (database, packing
  * matrix, tail keys, prefixes). */
 size_t alto_engine_device_bytes(const void *e);

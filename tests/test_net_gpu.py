@@ -144,7 +144,8 @@ def test_two_databases_on_two_ports_and_a_stop_frame(hip_device, dbs):
         e.close()
 
 
-def test_poisson_load_over_tcp(hip_device, dbs):
+@pytest.mark.parametrize("client", ["python", "native"])
+def test_poisson_load_over_tcp(hip_device, dbs, client):
     inp = dbs[0]
     eng = Engine.from_inputs(inp)
     srv = Server([eng], max_batch=64, slots=256, contexts=2)
@@ -155,8 +156,9 @@ def test_poisson_load_over_tcp(hip_device, dbs):
     frames.append(np.zeros(inp.request_len + 7, np.uint8))
     statuses = [workload.frame_status(f, inp.rows, inp.kp) for f in frames]
     want = {i: r.tobytes() for i, r in enumerate(oracle_responses(inp, frames[:16]))}
-    r = net.tcp_load("127.0.0.1", ns.port(0), frames, rate=400, seconds=2.0, conns=4, seed=63, drain_s=3,
-                     expect_status=statuses, expect=want)
+    run = net.tcp_load if client == "python" else net.native_load
+    r = run("127.0.0.1", ns.port(0), frames, rate=400 if client == "python" else 4000, seconds=2.0, conns=4, seed=63,
+            drain_s=3, expect_status=statuses, expect=want)
     assert r["missing"] == 0 and r["wrong_status"] == 0 and r["bad_responses"] == 0, r
     assert r["answered"] > 0 and r["checked_responses"] == r["answered"]
     ns.close()
