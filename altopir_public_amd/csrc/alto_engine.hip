@@ -1136,6 +1136,55 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
     });
 }
 
+// The offline hint from the engine's resident database (src/server.rs:656-709, 738-768): the
+// n x rows expansion of the query polynomials as limb rows (hint_limb_kernel), the scan
+// contraction with the exact mod-Q store, and the 128 S_z correction (hint_finish_kernel).
+int alto_engine_hint0(void *p, const uint64_t *a, size_t npolys, uint64_t *hint) {
+    auto *e = (Engine *)p;
+    if (!e || !a || !hint) {
+        set_err("alto_engine_hint0: null argument");
+        return -1;
+    }
+    const size_t n = e->n, C = e->cols, KC = e->KC, MT = n / 4;
+    if (n < 16 || (n & (n - 1)) || e->rows % n || npolys != e->rows / n) {
+        set_err("alto_engine_hint0: %zu polynomials for %zu records of n = %zu (records must be a multiple of n)",
+                npolys, e->rows, n);
+        return -1;
+    }
+    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    uint64_t *da = nullptr, *dout = nullptr;
+    int8_t *qa = nullptr;
+    unsigned long long *acc = nullptr, *ssum = nullptr;
+    hipStream_t st = nullptr;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+              HIP_OK(hipMalloc(&da, npolys * n * 8)) && HIP_OK(hipMalloc(&qa, MT * KC * kBlockBytes)) &&
+              HIP_OK(hipMalloc(&acc, n * C * 8)) && HIP_OK(hipMalloc(&ssum, n * 8)) &&
+              HIP_OK(hipMalloc(&dout, n * C * 8));
+    ok = ok && HIP_OK(hipMemcpyAsync(da, a, npolys * n * 8, hipMemcpyHostToDevice, st)) &&
+         HIP_OK(hipMemsetAsync(acc, 0, n * C * 8, st)) && HIP_OK(hipMemsetAsync(ssum, 0, n * 8, st));
+    if (ok) {
+        hipLaunchKernelGGL(hint_limb_kernel, dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)n), dim3(256), 0, st,
+                           (const uint64_t *)da, (int)n, KC, qa, ssum);
+        GemmArgs g{};
+        g.KC = (int)KC;
+        g.MT = (int)MT;
+        g.NT = (int)e->CT;
+        g.C = (int)C;
+        g.out64 = acc;
+        const Variant &v = kVariants[default_variant(0, (int)MT)];
+        v.prepare_scan();
+        v.scan(g, qa, e->db, st);
+        hipLaunchKernelGGL(hint_finish_kernel, dim3((unsigned)((n * C + 255) / 256)), dim3(256), 0, st, dout,
+                           (const unsigned long long *)acc, (const unsigned long long *)ssum, C, n * C);
+        ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(hint, dout, n * C * 8, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipStreamSynchronize(st));
+    }
+    for (void *q : {(void *)da, (void *)dout, (void *)qa, (void *)acc, (void *)ssum})
+        if (q) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
 int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *const *bold_t_bar,
                     const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg, size_t n, size_t rho,
                     uint64_t qmod, int threads) {

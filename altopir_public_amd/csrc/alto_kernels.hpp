@@ -410,6 +410,74 @@ __global__ __launch_bounds__(256) void prep_kernel(PrepArgs p) {
     for (size_t i = (size_t)b * blockDim.x + threadIdx.x; i < p.acc4; i += (size_t)p.nfill * blockDim.x) p.acc[i] = z;
 }
 
+// ------------------------------------------------------------- setup: the hint --
+
+// Query limbs of the offline hint (the reference's answer_hint_ring, src/server.rs:738-768,
+// with multiply_with_db_ring, :656-709): hint[z][col] = sum_r (a_r * d_{col,r})[z] in
+// Z_Q[x]/(x^n + 1), where d_{col,r}[j] is the byte of column col in record r n + j.  Over the
+// records this is a contraction with n queries z whose words are q_z[r n + j] = a_r[z - j] for
+// j <= z and -a_r[n + z - j] otherwise (mod Q).  Each word is centred (q' = q - Q above
+// 0x7F7F7F7F, so q' + 0x80808080 lies in [0, 2^32)) and split into four signed base-256 digits
+// whose weighted sum is q' exactly, the limb rows of query z in the scan's A-operand layout;
+// the exact sum S_z of the centred words goes to ssum[z] (two's complement).
+// Grid (ceil(KC * 4 / 256), n); n a power of two of at least 16, so the 16 words of a thread
+// lie in one polynomial.
+__global__ __launch_bounds__(256) void hint_limb_kernel(const uint64_t *__restrict__ a, int n, size_t KC,
+                                                        int8_t *__restrict__ dst,
+                                                        unsigned long long *__restrict__ ssum) {
+    __shared__ unsigned long long red[8];
+    const int z = (int)blockIdx.y;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t kc = p >> 2;
+    const int g = (int)(p & 3);
+    unsigned long long s = 0;
+    if (kc < KC) {
+        const size_t k0 = kc * 64 + (size_t)g * 16;
+        const size_t r = k0 / (size_t)n;
+        const int j0 = (int)(k0 - r * (size_t)n);
+        const uint64_t *ar = a + r * (size_t)n;
+        uint32_t w[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int j = j0 + e;
+            uint64_t v;
+            if (j <= z) {
+                v = ar[z - j] % kQ;
+            } else {
+                v = ar[n + z - j] % kQ;
+                v = v ? kQ - v : 0;
+            }
+            const int64_t c = v <= 0x7F7F7F7Full ? (int64_t)v : (int64_t)v - (int64_t)kQ;
+            s += (unsigned long long)c;
+            w[e] = (uint32_t)(c + 0x80808080ll);
+        }
+        uint4 o[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[i].x = pick_byte(w[0], w[1], w[2], w[3], i) ^ 0x80808080u;
+            o[i].y = pick_byte(w[4], w[5], w[6], w[7], i) ^ 0x80808080u;
+            o[i].z = pick_byte(w[8], w[9], w[10], w[11], i) ^ 0x80808080u;
+            o[i].w = pick_byte(w[12], w[13], w[14], w[15], i) ^ 0x80808080u;
+        }
+        uint4 *blk = (uint4 *)(dst + ((size_t)(z >> 2) * KC + kc) * kBlockBytes);
+#pragma unroll
+        for (int i = 0; i < 4; i++) blk[(z & 3) * 4 + i + 16 * g] = o[i];
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) atomicAdd(&ssum[z], s);
+}
+
+// hint[z * C + col] = acc + 128 S_z (mod Q): acc holds sum_k d'_k q'_k mod Q (d' = d - 128, the
+// database as stored), so adding 128 sum_k q'_k gives sum_k d_k q_k.
+__global__ void hint_finish_kernel(uint64_t *__restrict__ out, const unsigned long long *__restrict__ acc,
+                                   const unsigned long long *__restrict__ ssum, size_t C, size_t total) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int64_t S = (int64_t)ssum[t / C] % (int64_t)kQ;
+    const uint32_t sq = (uint32_t)(S < 0 ? S + (int64_t)kQ : S);
+    out[t] = addmod((uint32_t)(acc[t] % kQ), mulmod(sq, 128u));
+}
+
 // Writes the dimensions {B, MT} of the batch a bucket-sized launch chain answers: the first
 // node of a captured chain, whose arguments each replay sets to the batch's own.
 __global__ void batch_dims_kernel(int *__restrict__ dims, int B, int MT) {
@@ -471,6 +539,10 @@ struct GemmArgs {
     // batch dimensions in device memory {B, MT} (a launch sized for a larger batch of the same
     // bucket, from a captured graph), or null: MT above is the batch's own
     const int *dims;
+    // scan only: when set, the exact limb sum of each (query, column) goes here mod Q (added
+    // when split) instead of mod 2^32 to `out` (the offline hint, whose query limbs sum to the
+    // centred words exactly)
+    unsigned long long *out64;
 };
 // A tiles of the batch: from the device dimensions when given, else the launch's own MT.
 __device__ __forceinline__ int batch_tiles(const GemmArgs &g) { return g.dims ? g.dims[1] : g.MT; }
@@ -493,6 +565,13 @@ template <bool ATOMIC>
 __device__ __forceinline__ void scan_store(const GemmArgs &g, int rt, int ct, i32x4 a) {
     const int lane = threadIdx.x & 63;
     const int c = 4 * rt + (lane >> 4), col = 16 * ct + (lane & 15);
+    if (g.out64) {  // sum_i 256^i P_i exactly (|P_i| <= 2^30 per split), reduced mod Q
+        const int64_t t = (int64_t)a.x + ((int64_t)a.y << 8) + ((int64_t)a.z << 16) + ((int64_t)a.w << 24);
+        unsigned long long *d = g.out64 + (size_t)c * g.C + col;
+        if (ATOMIC) atomicAdd(d, (unsigned long long)red_i64(t));
+        else *d = red_i64(t);
+        return;
+    }
     uint32_t *dst = g.out + (size_t)c * g.C + col;
     if (ATOMIC) atomicAdd(dst, recombine_u32(a));
     else *dst = recombine_u32(a);

@@ -256,6 +256,17 @@ class Engine:
             raise RuntimeError(f"engine error {r} ({_lib.last_error()})")
         return r, list(stage)
 
+    def hint0(self, a: np.ndarray) -> np.ndarray:
+        """The offline hint from the resident database (alto_engine_hint0; src/server.rs:656-709,
+        738-768): a is (rows / n, n) uint64, the query polynomials in coefficient form; returns
+        (n, cols) uint64 values mod Q."""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        n = a.shape[1]
+        out = np.zeros((n, self.cols), np.uint64)
+        if self._lib.alto_engine_hint0(self.ptr, a.ctypes.data, a.shape[0], out.ctypes.data) != 0:
+            raise RuntimeError(f"alto_engine_hint0 failed ({_lib.last_error()})")
+        return out
+
     def answer_device(self, slots_ptr: int, slot_stride: int, b: int, out_ptr: int, out_stride: int,
                       status_ptr: int, stream: int = 0) -> int:
         """Device-resident batch (alto_answer_device): all pointers are device addresses."""

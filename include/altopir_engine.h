@@ -63,6 +63,17 @@ int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
                     const uint32_t *tables, const uint32_t *gen_pows, size_t num_iter, size_t tg, size_t n, size_t rho,
                     uint64_t qmod, int threads);
 
+/* No counterpart in the reference's C ABI: the offline hint of the reference's setup
+ * (YServer::answer_hint_ring, src/server.rs:738-768, with multiply_with_db_ring, :656-709)
+ * computed on the device from the engine's resident database:
+ *   hint[z * cols + col] = sum_r (a_r * d_{col,r})[z]  in Z_Q[x]/(x^n + 1),  z < n,
+ * where d_{col,r}[j] is the byte of column col in record r * n + j and a_r (n coefficients,
+ * a[r * n + j]) is query polynomial r in coefficient form: the reference's
+ * query_raw_transformed (after negacyclic_perm, before its NTT), reduced mod Q.  a: npolys =
+ * rows / n polynomials (host); hint: n * cols values < Q (host).  Returns 0, or -1
+ * (alto_last_error says why) when rows is not npolys * n or a device call fails. */
+int alto_engine_hint0(void *e, const uint64_t *a, size_t npolys, uint64_t *hint);
+
 /* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
  * Returns nullptr for the configurations the reference rejects (Q != 4294955009,
  * n == 0 or n > 4096 or n % 16, cols % n, rows % 64, cols % 32, threads <= 0,

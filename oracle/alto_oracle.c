@@ -429,3 +429,29 @@ void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, 
             out[i * bc + c] += s;
         }
 }
+
+/* The offline hint, src/server.rs:656-709 (multiply_with_db_ring) for the columns
+ * [col0, col0 + ncols): for each column, the sum over the record polynomials r of the product
+ * of query polynomial a_r (a[r n .. r n + n), after negacyclic_perm, in coefficient form) with
+ * the column's record bytes d_r[j] = db_col[col * ld + r n + j], in Z_Q[x]/(x^n + 1); the
+ * reference multiplies in the NTT domain and returns the coefficients, here the negacyclic
+ * product is summed term by term (x^{i + j} = -x^{i + j - n} past degree n - 1).
+ * hint[z * stride + (col - col0)] receives coefficient z (src/server.rs:702-707 transposes
+ * the per-column results into z-major order). */
+void orc_hint0(uint64_t *hint, size_t stride, const uint64_t *a, size_t npolys, size_t n, const uint8_t *db_col,
+               size_t ld, size_t col0, size_t ncols) {
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (size_t cc = 0; cc < ncols; cc++) {
+        for (size_t z = 0; z < n; z++) {
+            const uint8_t *d = db_col + (col0 + cc) * ld;
+            uint64_t pos = 0, neg = 0; /* each < 2^61: R terms of at most 2^40 */
+            for (size_t r = 0; r < npolys; r++) {
+                const uint64_t *ar = a + r * n;
+                const uint8_t *dr = d + r * n;
+                for (size_t j = 0; j <= z; j++) pos += (ar[z - j] % ORC_Q) * dr[j];
+                for (size_t j = z + 1; j < n; j++) neg += (ar[n + z - j] % ORC_Q) * dr[j];
+            }
+            hint[z * stride + cc] = (pos % ORC_Q + ORC_Q - neg % ORC_Q) % ORC_Q;
+        }
+    }
+}

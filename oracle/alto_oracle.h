@@ -31,6 +31,9 @@
  *                     the coefficient polynomial
  *   orc_answer        compiled reference, byte for byte, on seeded inputs
  *   orc_matmul_vec_packed  compiled reference (src/matmul.cpp) and brute force
+ *   orc_hint0         an independent evaluation (full linear convolution of each pair of
+ *                     polynomials, then the fold x^n = -1), and the closed forms for the query
+ *                     polynomials 1 and x (sums of record bytes, shifted with one negated term)
  */
 #ifndef ALTO_ORACLE_H
 #define ALTO_ORACLE_H
@@ -94,6 +97,8 @@ void orc_pack_online(uint64_t *beta, const uint32_t *y, const uint64_t *const *b
 
 void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t rows, size_t cols,
                            size_t bc);
+void orc_hint0(uint64_t *hint, size_t stride, const uint64_t *a, size_t npolys, size_t n, const uint8_t *db_col,
+               size_t ld, size_t col0, size_t ncols);
 
 /* Forward negacyclic NTT under the same slot convention, used only to pin orc_intt. */
 void orc_ntt_naive(uint64_t *out, const uint64_t *in, size_t n, uint64_t psi);

@@ -67,6 +67,7 @@ def oracle() -> C.CDLL:
         lib.orc_pack_build.argtypes = [P, P, P, P, P, SZ, SZ, SZ, SZ]
         lib.orc_tile_db.argtypes = [P, P, SZ, SZ, SZ]
         lib.orc_pack_online.argtypes = [P, P, P, P, P, P, SZ, SZ, SZ, SZ]
+        lib.orc_hint0.argtypes = [P, SZ, P, SZ, SZ, P, SZ, SZ, SZ]
         lib.orc_ntt_naive.argtypes = [P, P, SZ, C.c_uint64]
         lib.orc_intt_naive.argtypes = [P, P, SZ, C.c_uint64]
         lib.orc_powmod.restype = C.c_uint64
@@ -214,3 +215,15 @@ class RefEngine:
 
     def __del__(self):
         self.close()
+
+
+def oracle_hint0(a: np.ndarray, db_col: np.ndarray, ld: int, n: int, col0: int = 0, ncols: int | None = None):
+    """orc_hint0: (n, ncols) uint64 hint columns col0 .. col0 + ncols of the column-major database."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    db_col = np.ascontiguousarray(db_col, dtype=np.uint8)
+    npolys = a.size // n
+    if ncols is None:
+        ncols = db_col.size // ld - col0
+    out = np.zeros((n, ncols), np.uint64)
+    oracle().orc_hint0(out.ctypes.data, ncols, a.ctypes.data, npolys, n, db_col.ctypes.data, ld, col0, ncols)
+    return out

@@ -5,7 +5,7 @@
     64 GiB (2,097,152 records)  batches 128, 1024          parts 0 and 15
 
 plus the three resident databases of the serving workload (1, 16 and 32 GiB) under Poisson
-load with corrupted frames.
+load with corrupted frames, and whole columns of the offline hint of the 1 and 64 GiB databases.
 
 The oracle cannot answer these batches whole in test time, but two properties of the path
 make the check exact without it: a response depends only on its own request (the batch only
@@ -306,3 +306,21 @@ def test_serving_three_resident_databases(hip_device):
     assert s["answered"] == sum(1 for x in want_status if x == 0)
     assert sum(s["rejected"].values()) == sum(1 for x in want_status if x != 0) >= 1
     assert s["p99_ms"] >= s["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("rows, cols_checked", [(32768, (0, 16383, 32767)), (2097152, (12345, 32767))])
+def test_full_size_hint_columns_match_the_oracle(hip_device, rows, cols_checked):
+    # the offline hint (src/server.rs:656-709, 738-768) of the 1 GiB and the 64 GiB database:
+    # 2048 x records x 32,768 columns on the device; whole columns of it against the oracle
+    Q = 4294955009
+    db = FullDb.get(rows)
+    rng = np.random.default_rng(rows)
+    a = rng.integers(0, Q, size=rows, dtype=np.uint64)
+    t0 = time.perf_counter()
+    hint = db.eng.hint0(a.reshape(rows // N, N))
+    log(f"{rows} records: hint of 2048 x {COLS} in {(time.perf_counter() - t0) * 1e3:.0f} ms (with its copies)")
+    for c in cols_checked:
+        col = db.d_db[c * rows:(c + 1) * rows].cpu().numpy()  # column c of the column-major database
+        want = oc.oracle_hint0(a, col, rows, N, 0, 1)
+        assert np.array_equal(hint[:, c], want[:, 0]), f"column {c}"
+    del hint

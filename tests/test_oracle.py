@@ -471,3 +471,52 @@ def test_matmul_vec_packed_matches_the_compiled_reference():
         f.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 2
         f(o2.ctypes.data, a.ctypes.data, b.ctypes.data, rows, cols)
         assert np.array_equal(o1, o2), fn
+
+
+def _hint_by_convolution(a, db_col, ld, n, cols):
+    # an independent evaluation of the hint: the full linear convolution of each pair of
+    # polynomials (numpy), folded with x^n = -1, summed over the records, mod Q
+    Q = 4294955009
+    npolys = a.size // n
+    out = np.zeros((n, cols), np.uint64)
+    for col in range(cols):
+        acc = np.zeros(n, dtype=object)
+        for r in range(npolys):
+            c = np.convolve(a[r * n:(r + 1) * n].astype(object), db_col[col * ld + r * n: col * ld + (r + 1) * n].astype(object))
+            folded = c[:n].copy()
+            folded[:n - 1] -= c[n:]
+            acc += folded
+        out[:, col] = np.array([int(v) % Q for v in acc], dtype=np.uint64)
+    return out
+
+
+@pytest.mark.parametrize("n, npolys, cols, ld_pad", [(16, 1, 5, 0), (32, 3, 4, 7), (64, 2, 3, 64)])
+def test_hint_matches_an_independent_convolution(n, npolys, cols, ld_pad):
+    Q = 4294955009
+    rng = np.random.default_rng(n + npolys)
+    rows = n * npolys
+    ld = rows + ld_pad
+    db_col = rng.integers(0, 256, size=ld * cols, dtype=np.uint8)
+    a = rng.integers(0, Q, size=rows, dtype=np.uint64)
+    a[:6] = [0, Q - 1, 0x7F7F7F7F, 0x7F7F7F80, 1, Q // 2]  # the centring boundaries and extremes
+    got = oc.oracle_hint0(a, db_col, ld, n)
+    assert np.array_equal(got, _hint_by_convolution(a, db_col, ld, n, cols))
+
+
+def test_hint_of_the_polynomials_one_and_x():
+    # a_r = 1: coefficient z is the sum over r of byte r n + z; a_r = x: the bytes shift up by one
+    # and the last one wraps around negated (x * x^{n-1} = -1)
+    Q = 4294955009
+    n, npolys, cols = 32, 4, 3
+    rng = np.random.default_rng(5)
+    db_col = rng.integers(0, 256, size=n * npolys * cols, dtype=np.uint8)
+    d = db_col.reshape(cols, npolys, n).astype(np.int64)
+    one = np.zeros(n * npolys, np.uint64)
+    one[::n] = 1
+    assert np.array_equal(oc.oracle_hint0(one, db_col, n * npolys, n), d.sum(axis=1).T.astype(np.uint64))
+    x = np.zeros(n * npolys, np.uint64)
+    x[1::n] = 1
+    want = np.zeros((n, cols), np.int64)
+    want[1:] = d[:, :, :-1].sum(axis=1).T
+    want[0] = -d[:, :, -1].sum(axis=1)
+    assert np.array_equal(oc.oracle_hint0(x, db_col, n * npolys, n), (want % Q).astype(np.uint64))

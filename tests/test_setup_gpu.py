@@ -79,3 +79,41 @@ def test_device_pack_build_at_full_size(hip_device):
     print(f"[setup] packing matrix n={n} tg={tg} rho={rho}: host sw_pack_build (16 threads) {t_host:.3f} s, "
           f"alto_pack_build {t_dev:.3f} s (host inputs in, host matrix out)", file=sys.stderr, flush=True)
     assert np.array_equal(dev, host)
+
+
+@pytest.mark.parametrize("rows,cols,n,check", [(1024, 96, 32, None), (4096, 512, 256, None), (8192, 4096, 2048, 64),
+                                               (131072, 256, 256, None)])
+def test_device_hint_matches_the_oracle(hip_device, rows, cols, n, check):
+    # the offline hint (src/server.rs:656-709, 738-768) from the engine's resident database:
+    # 8 to 512 A tiles of query coefficients (8-row and 16-row contraction shapes); 131,072
+    # records = 2,048 k-steps, so the record range is split and the exact sums are added
+    from altopir_public_amd.engine import Engine
+
+    Q = 4294955009
+    inp = workload.make_inputs(rows, cols, n=n, tg=4, seed=rows + cols + n)
+    eng = Engine.from_inputs(inp)
+    rng = np.random.default_rng(rows + n)
+    a = rng.integers(0, Q, size=rows, dtype=np.uint64)
+    a[:6] = [0, Q - 1, 0x7F7F7F7F, 0x7F7F7F80, 1, Q // 2]  # the centring boundaries and extremes
+    t0 = time.perf_counter()
+    got = eng.hint0(a.reshape(rows // n, n))
+    dt = time.perf_counter() - t0
+    eng.close()
+    print(f"[setup] hint {rows} records x {cols} columns, n = {n}: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    assert got.shape == (n, cols) and (got < Q).all()
+    if check is None:
+        assert np.array_equal(got, oc.oracle_hint0(a, inp.db_col, inp.ld, n))
+    else:  # the first and the last `check` columns
+        for c0 in (0, cols - check):
+            want = oc.oracle_hint0(a, inp.db_col, inp.ld, n, c0, check)
+            assert np.array_equal(got[:, c0:c0 + check], want), f"columns {c0}..{c0 + check}"
+
+
+def test_device_hint_rejects_a_record_count_that_is_not_whole_polynomials(hip_device):
+    from altopir_public_amd.engine import Engine
+
+    inp = workload.make_inputs(2048 + 64, 2048, n=2048, tg=4, seed=3)
+    eng = Engine.from_inputs(inp)
+    with pytest.raises(RuntimeError, match="multiple of n"):
+        eng.hint0(np.zeros((1, 2048), np.uint64))
+    eng.close()
