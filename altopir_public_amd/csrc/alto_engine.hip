@@ -1136,22 +1136,35 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
     });
 }
 
-// The offline hint from the engine's resident database (src/server.rs:656-709, 738-768): the
-// n x rows expansion of the query polynomials as limb rows (hint_limb_kernel), the scan
-// contraction with the exact mod-Q store, and the 128 S_z correction (hint_finish_kernel).
-int alto_engine_hint0(void *p, const uint64_t *a, size_t npolys, uint64_t *hint) {
-    auto *e = (Engine *)p;
-    if (!e || !a || !hint) {
-        set_err("alto_engine_hint0: null argument");
-        return -1;
+}  // extern "C"
+
+namespace {
+
+// A column-major database (`ld`-strided columns of `rows` bytes, host or device memory) into
+// the tiled layout at dst, through `stage` (stage_cols columns at a time).
+bool tile_db_columns(const uint8_t *db_col, bool on_device, size_t ld, size_t rows, size_t cols, int8_t *dst,
+                     uint8_t *stage, size_t stage_cols, hipStream_t st) {
+    const size_t KC = rows / 64;
+    bool ok = true;
+    for (size_t c0 = 0; ok && c0 < cols; c0 += stage_cols) {
+        const size_t nc = std::min(stage_cols, cols - c0);
+        ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc,
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        const size_t work = nc / 16 * KC * 64;
+        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, stage, rows, dst,
+                           nc / 16, KC, c0 / 16);
+        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(st));
     }
-    const size_t n = e->n, C = e->cols, KC = e->KC, MT = n / 4;
-    if (n < 16 || (n & (n - 1)) || e->rows % n || npolys != e->rows / n) {
-        set_err("alto_engine_hint0: %zu polynomials for %zu records of n = %zu (records must be a multiple of n)",
-                npolys, e->rows, n);
-        return -1;
-    }
-    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    return ok;
+}
+
+// The offline hint (src/server.rs:656-709, 738-768) of a tiled database of `rows` records x
+// `cols` columns on the current device: the n x rows expansion of the query polynomials as
+// limb rows (hint_limb_kernel), the scan contraction with the exact mod-Q store, and the
+// 128 S_z correction (hint_finish_kernel).
+bool hint_from_tiled(const int8_t *db, size_t rows, size_t cols, size_t n, const uint64_t *a, size_t npolys,
+                     uint64_t *hint) {
+    const size_t C = cols, KC = rows / 64, MT = n / 4;
     uint64_t *da = nullptr, *dout = nullptr;
     int8_t *qa = nullptr;
     unsigned long long *acc = nullptr, *ssum = nullptr;
@@ -1168,12 +1181,12 @@ int alto_engine_hint0(void *p, const uint64_t *a, size_t npolys, uint64_t *hint)
         GemmArgs g{};
         g.KC = (int)KC;
         g.MT = (int)MT;
-        g.NT = (int)e->CT;
+        g.NT = (int)(C / 16);
         g.C = (int)C;
         g.out64 = acc;
         const Variant &v = kVariants[default_variant(0, (int)MT)];
         v.prepare_scan();
-        v.scan(g, qa, e->db, st);
+        v.scan(g, qa, db, st);
         hipLaunchKernelGGL(hint_finish_kernel, dim3((unsigned)((n * C + 255) / 256)), dim3(256), 0, st, dout,
                            (const unsigned long long *)acc, (const unsigned long long *)ssum, C, n * C);
         ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(hint, dout, n * C * 8, hipMemcpyDeviceToHost, st)) &&
@@ -1181,6 +1194,54 @@ int alto_engine_hint0(void *p, const uint64_t *a, size_t npolys, uint64_t *hint)
     }
     for (void *q : {(void *)da, (void *)dout, (void *)qa, (void *)acc, (void *)ssum})
         if (q) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+    return ok;
+}
+
+bool hint_shape_ok(const char *fn, size_t rows, size_t cols, size_t n, size_t npolys) {
+    if (n < 16 || (n & (n - 1)) || !rows || rows % 64 || !cols || cols % 16 || rows % n || npolys != rows / n ||
+        rows / 64 > (size_t)INT32_MAX || cols > (size_t)INT32_MAX) {
+        set_err("%s: %zu polynomials of n = %zu for %zu records x %zu columns (n a power of two >= 16, records a "
+                "multiple of n and of 64, columns of 16)", fn, npolys, n, rows, cols);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alto_engine_hint0(void *p, const uint64_t *a, size_t npolys, uint64_t *hint) {
+    auto *e = (Engine *)p;
+    if (!e || !a || !hint) {
+        set_err("alto_engine_hint0: null argument");
+        return -1;
+    }
+    if (!hint_shape_ok("alto_engine_hint0", e->rows, e->cols, e->n, npolys)) return -1;
+    if (!HIP_OK(hipSetDevice(e->device))) return -1;
+    return hint_from_tiled(e->db, e->rows, e->cols, e->n, a, npolys, hint) ? 0 : -1;
+}
+
+int alto_hint0(const uint8_t *db_col, int on_device, size_t ld, size_t rows, size_t cols, size_t n, const uint64_t *a,
+               size_t npolys, uint64_t *hint) {
+    if (!db_col || !a || !hint || ld < rows) {
+        set_err("alto_hint0: null argument or ld < rows");
+        return -1;
+    }
+    if (!hint_shape_ok("alto_hint0", rows, cols, n, npolys)) return -1;
+    int dev = 0;
+    if (!HIP_OK(hipGetDevice(&dev))) return -1;
+    int8_t *db = nullptr;
+    uint8_t *stage = nullptr;
+    hipStream_t st = nullptr;
+    const size_t stage_cols = std::max<size_t>(16, std::min<size_t>(cols, ((size_t)256 << 20) / rows / 16 * 16));
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && HIP_OK(hipMalloc(&db, rows * cols)) &&
+              HIP_OK(hipMalloc(&stage, stage_cols * rows));
+    ok = ok && tile_db_columns(db_col, on_device != 0, ld, rows, cols, db, stage, stage_cols, st);
+    if (stage) (void)hipFree(stage);
+    ok = ok && hint_from_tiled(db, rows, cols, n, a, npolys, hint);
+    if (db) (void)hipFree(db);
     if (st) (void)hipStreamDestroy(st);
     return ok ? 0 : -1;
 }
@@ -1306,15 +1367,7 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
                            setup, (const uint8_t *)stage, cols, e->db, e->CT, e->KC, r0 / 64);
         ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(setup));
     }
-    for (size_t c0 = 0; ok && !db_rows && c0 < cols; c0 += stage_cols) {
-        const size_t nc = std::min(stage_cols, cols - c0);
-        ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc,
-                                     db_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, setup));
-        const size_t work = nc / 16 * e->KC * 64;
-        hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, setup, stage,
-                           (size_t)rows, e->db, nc / 16, e->KC, c0 / 16);
-        ok = ok && HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(setup));
-    }
+    if (ok && !db_rows) ok = tile_db_columns(db_col, db_on_device, ld, rows, cols, e->db, stage, stage_cols, setup);
     // packing matrix: one byte plane at a time
     for (int j = 0; ok && j < 4; j++) {
         ok = HIP_OK(hipMemcpyAsync(stage, ub + (size_t)j * e->kp * cols, e->kp * cols, hipMemcpyHostToDevice, setup));

@@ -118,6 +118,19 @@ class TiledDb:
             self.ptr = None
 
 
+def hint0(db_col, ld: int, rows: int, cols: int, n: int, a: np.ndarray, on_device: bool = False) -> np.ndarray:
+    """The offline hint (src/server.rs:656-709, 738-768) of a column-major database without an
+    engine (alto_hint0): db_col is a host uint8 array (column col at col * ld) or, with
+    on_device, a device address; a is (rows / n, n) uint64.  Returns (n, cols) uint64 mod Q."""
+    lib = _lib.load()
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    out = np.zeros((n, cols), np.uint64)
+    ptr = int(db_col) if on_device else _ptr(db_col)
+    if lib.alto_hint0(ptr, int(on_device), ld, rows, cols, n, a.ctypes.data, a.size // n, out.ctypes.data) != 0:
+        raise RuntimeError(f"alto_hint0 failed ({_lib.last_error()})")
+    return out
+
+
 def build_packing_matrix(ub: np.ndarray, bold_t: Sequence[np.ndarray], bold_t_bar: Sequence[np.ndarray],
                          tables: np.ndarray, gen_pows: np.ndarray, num_iter: int, tg: int, n: int, qmod: int,
                          threads: int, device: bool = False) -> None:

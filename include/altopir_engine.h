@@ -73,6 +73,13 @@ int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
  * rows / n polynomials (host); hint: n * cols values < Q (host).  Returns 0, or -1
  * (alto_last_error says why) when rows is not npolys * n or a device call fails. */
 int alto_engine_hint0(void *e, const uint64_t *a, size_t npolys, uint64_t *hint);
+/* The same hint from a column-major database (column col at db_col + col * ld, rows bytes; host
+ * memory, or device memory when on_device != 0) on the current device, without an engine (the
+ * reference computes the hint before it can build the packing matrix an engine needs): the
+ * columns are tiled on the device in slices, then as above.  rows a multiple of n and of 64,
+ * cols of 16.  Returns 0, or -1 (alto_last_error says why). */
+int alto_hint0(const uint8_t *db_col, int on_device, size_t ld, size_t rows, size_t cols, size_t n, const uint64_t *a,
+               size_t npolys, uint64_t *hint);
 
 /* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
  * Returns nullptr for the configurations the reference rejects (Q != 4294955009,

@@ -117,3 +117,25 @@ def test_device_hint_rejects_a_record_count_that_is_not_whole_polynomials(hip_de
     with pytest.raises(RuntimeError, match="multiple of n"):
         eng.hint0(np.zeros((1, 2048), np.uint64))
     eng.close()
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_hint_without_an_engine_matches_the_oracle(hip_device, on_device):
+    # alto_hint0 tiles a column-major database (host or device memory, padded columns) itself
+    import torch
+
+    from altopir_public_amd.engine import hint0
+
+    Q = 4294955009
+    rows, cols, n, ld = 4096, 512, 256, 4096 + 64
+    rng = np.random.default_rng(70 + on_device)
+    db_col = rng.integers(0, 256, size=ld * cols, dtype=np.uint8)
+    a = rng.integers(0, Q, size=rows, dtype=np.uint64)
+    if on_device:
+        d = torch.from_numpy(db_col).cuda()
+        torch.cuda.synchronize()
+        got = hint0(d.data_ptr(), ld, rows, cols, n, a.reshape(-1, n), on_device=True)
+        del d
+    else:
+        got = hint0(db_col, ld, rows, cols, n, a.reshape(-1, n))
+    assert np.array_equal(got, oc.oracle_hint0(a, db_col, ld, n))
