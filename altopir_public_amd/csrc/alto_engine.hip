@@ -192,6 +192,30 @@ void launch_dmab(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
                            st, (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+constexpr size_t dmac_lds() {
+    return DmacShape<TM, TN, WM, WN, D, UNR, PACK, false>::kLds;
+}
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+void prepare_dmac() {
+    constexpr size_t lds = dmac_lds<TM, TN, WM, WN, D, UNR, PACK>();
+    (void)hipFuncSetAttribute((const void *)dmac_kernel<TM, TN, WM, WN, D, UNR, PACK, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)dmac_kernel<TM, TN, WM, WN, D, UNR, PACK, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+void launch_dmac(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
+    constexpr size_t lds = dmac_lds<TM, TN, WM, WN, D, UNR, PACK>();
+    const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
+    if (g.gz > 1)
+        hipLaunchKernelGGL((dmac_kernel<TM, TN, WM, WN, D, UNR, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+    else
+        hipLaunchKernelGGL((dmac_kernel<TM, TN, WM, WN, D, UNR, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds,
+                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+}
+
 // Kernel variants of a contraction.  Every variant computes the full result for any number
 // of A tiles (ragged tiles are clamped and masked); they differ in tile shapes and in how
 // operands are staged, which the tuner chooses per batch size.
@@ -274,6 +298,9 @@ const Variant kVariants[] = {
      prepare_dmab<8, 4, 2, 4, 2, 8, false>, prepare_dmab<8, 4, 2, 4, 2, 8, true>},
     {"dmab8_m8_n4_u16", launch_dmab<8, 4, 2, 4, 2, 16, false>, launch_dmab<8, 4, 2, 4, 2, 16, true>,
      prepare_dmab<8, 4, 2, 4, 2, 16, false>, prepare_dmab<8, 4, 2, 4, 2, 16, true>},
+    // the same with the A fragments read from LDS just before their MFMAs
+    {"dmac8_m8_n4_d2u16", launch_dmac<8, 4, 2, 4, 2, 16, false>, launch_dmac<8, 4, 2, 4, 2, 16, true>,
+     prepare_dmac<8, 4, 2, 4, 2, 16, false>, prepare_dmac<8, 4, 2, 4, 2, 16, true>},
     // the same with 12 x 16 and 8 x 32-tile workgroups (the 8 x 32 one with one wave per
     // column of four B tiles), for 17-48 queries
     {"dmab8_m6_n4_u16", launch_dmab<6, 4, 2, 4, 2, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
@@ -357,6 +384,29 @@ struct FusedRegB {
     }
 };
 using FusedRegB16 = FusedRegB<8, 4, 2, 4, 2, 16>;
+// Scan on the just-in-time-fragment form, packing on the register-streaming 16 x 16 tiles.
+struct FusedRegC16 {
+    using SA = DmacShape<8, 4, 2, 4, 2, 16, false, true>;
+    using SN = DmacShape<8, 4, 2, 4, 2, 16, false, false>;
+    using P = DmabShape<8, 4, 2, 4, 2, 16, true, false>;
+    static constexpr size_t kLds = SA::kLds > P::kLds ? SA::kLds : P::kLds;
+    static void prepare() {
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SA, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SN, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLds);
+    }
+    static void launch(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya,
+                       const int8_t *u, hipStream_t st) {
+        const int sb = plan_ring<8, 4, 2, 4>(gs, false), pb = plan_ring<8, 4, 2, 4>(gp, true);
+        if (gs.gz > 1)
+            hipLaunchKernelGGL((scan_pack_kernel<SA, P>), dim3(sb + pb), dim3(SA::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+        else
+            hipLaunchKernelGGL((scan_pack_kernel<SN, P>), dim3(sb + pb), dim3(SN::kThreads), kLds, st,
+                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
+    }
+};
 using FusedRing8 = FusedRing<4, 8, 4, 2, 4, 1, 4, 8, 4, 2, 4, 1>;
 using FusedRingWide = FusedRing<8, 6, 2, 4, 4, 2, 4, 8, 4, 2, 4, 1>;
 
@@ -372,6 +422,7 @@ const FusedPair kFused[] = {
     {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
     {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
     {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", FusedRegB16::launch, FusedRegB16::prepare},
+    {"dmac8_m8_n4_d2u16", "dmab8_m8_n4_u16", FusedRegC16::launch, FusedRegC16::prepare},
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
@@ -386,18 +437,22 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 // contractions on the 16 x 16-tile workgroups that stream the B blocks into registers, 16
 // k-steps per loop iteration (at batch 128: scan 0.432 against 0.492 ms for the best all-LDS
 // ring at 1 GiB, 30.3 against 30.6 ms at 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB
-// batch 2048 59.2 against 60.1 ms).
+// batch 2048 59.2 against 60.1 ms).  A scan of more than 48 queries over a record range that
+// runs unsplit (KC <= kFoldSteps, at most 65,536 records) takes the form that reads its A
+// fragments just before their MFMAs: 3-10% faster at 1 GiB and 5-9% at 2 GiB (batches of 64 and
+// 128), while with two or more splits it was 1-5% slower (4 and 64 GiB at batch 128).
 int variant_index(const char *name);
-int default_variant(int which, int MT) {
+int default_variant(int which, int MT, int KC) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
                      rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
-                     regb = variant_index("dmab8_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16");
+                     regb = variant_index("dmab8_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
+                     regc = variant_index("dmac8_m8_n4_d2u16");
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
     if (MT <= 8) return rows8;
     if (MT <= 12) return rows12;
-    return regb;
+    return KC <= kFoldSteps ? regc : regb;
 }
 int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)
@@ -457,7 +512,7 @@ struct Engine {
     ~Engine();
     int variant_for(int which, int MT) const {
         const int v = tuned[which][bucket_of(MT)];
-        return v >= 0 ? v : default_variant(which, MT);
+        return v >= 0 ? v : default_variant(which, MT, (int)(which ? KCp : KC));
     }
     // kFused index used for a batch of MT A tiles, or -1 for two launches
     // Built-in: fused only for batches of one A tile (1-4 queries) on databases of at most
@@ -999,7 +1054,7 @@ int packed_product(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t r
         g.NT = (int)CT;
         g.C = (int)R16;
         g.out = acc;
-        const Variant &v = kVariants[default_variant(0, (int)MT)];
+        const Variant &v = kVariants[default_variant(0, (int)MT, (int)KC)];
         v.prepare_scan();  // the ring kernels' LDS size, which engine set-up otherwise sets
         v.scan(g, qa, db, st);
         hipLaunchKernelGGL(packed_finish_kernel, dim3((unsigned)((rows * bc + 255) / 256)), dim3(256), 0, st, dout,
@@ -1184,7 +1239,7 @@ bool hint_from_tiled(const int8_t *db, size_t rows, size_t cols, size_t n, const
         g.NT = (int)(C / 16);
         g.C = (int)C;
         g.out64 = acc;
-        const Variant &v = kVariants[default_variant(0, (int)MT)];
+        const Variant &v = kVariants[default_variant(0, (int)MT, (int)KC)];
         v.prepare_scan();
         v.scan(g, qa, db, st);
         hipLaunchKernelGGL(hint_finish_kernel, dim3((unsigned)((n * C + 255) / 256)), dim3(256), 0, st, dout,

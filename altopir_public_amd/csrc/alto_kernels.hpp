@@ -1158,6 +1158,121 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void dmab_kernel(const i32x4 *__re
     dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a_src, b_src, g, blockIdx.x);
 }
 
+// The register-streaming contraction with the A fragments read from LDS just before their
+// MFMAs (two in flight) instead of a whole k-step ahead: 24 fewer registers per wave (a deeper
+// B register ring, D = 3 or 4, fits in them, but measured no faster than D = 2).  Loads are
+// issued in pairs P(t) = (A(t) into ring slot t % S, B(t) into register slot t % D), D pairs
+// ahead; at the top of step k P(k) must have landed ((D - 1) pairs stay in flight), and the
+// barrier then publishes A(k) and frees the slot of step k - 1, into which A(k + D) goes at
+// the end of step k (S = D + 1 slots).  Steps past the end of the range work as in dmab_tile.
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+__device__ __forceinline__ void dmac_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
+                                          const GemmArgs &g, int L) {
+    constexpr int NW = WM * WN;
+    static_assert(NW == 8, "two waves per SIMD");
+    constexpr int BM = WM * TM;
+    static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    static_assert(D >= 2 && D <= 4, "prefetch distance");
+    static_assert(UNR % D == 0, "whole register rings per loop iteration");
+    constexpr int S = D + 1;
+    constexpr int PA = (BM + NW - 1) / NW;  // A blocks each wave copies per step
+    constexpr int BMP = PA * NW;            // A blocks per ring slot
+    constexpr int PAIR = PA + TN;           // vector-memory instructions per step and wave
+    static_assert(D * PAIR < 64, "vmcnt range");
+    constexpr int BS = b_kstride<PACK>();
+    extern __shared__ i32x4 ring[];         // [S][BMP][64]
+    const TileMap tm = map_tile_at(L, g.gx, g.gy, g.gz);
+    if (!tm.valid) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave / WN, wn = wave % WN;
+    const int mt0 = tm.y * BM, bt0 = (tm.x * WN + wn) * TN;
+    const int MT = batch_tiles(g);
+    if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
+    const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    const i32x4 *asrc[PA];
+#pragma unroll
+    for (int p = 0; p < PA; p++) asrc[p] = a_src + (size_t)min(mt0 + wave + NW * p, MT - 1) * g.KC * 64 + lane;
+    const i32x4 *bsrc[TN];
+#pragma unroll
+    for (int n = 0; n < TN; n++) bsrc[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
+    const i32x4 *zero = kZeroBlock + lane;
+    auto issue_a = [&](int t) {  // A(t) into its slot (t past the range: the last block)
+        i32x4 *dst = ring + (size_t)(((t - kb) % S) * BMP) * 64;
+        const size_t kk = (size_t)min(t, ke - 1);
+#pragma unroll
+        for (int p = 0; p < PA; p++)
+            __builtin_amdgcn_global_load_lds((const void *)(asrc[p] + kk * 64), (lds_void *)(dst + (wave + NW * p) * 64),
+                                             16, 0, 0);
+    };
+    auto load_b = [&](int t, i32x4 *b) {  // B(t) (t past the range: zeros)
+        const bool live = t < ke;
+#pragma unroll
+        for (int n = 0; n < TN; n++) b[n] = *(live ? bsrc[n] + (size_t)t * BS : zero);
+    };
+    i32x4 acc[TM][TN];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+    if (ke > kb) {
+        i32x4 breg[D][TN];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            issue_a(kb + d);
+            load_b(kb + d, breg[d]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const int kend = kb + (ke - kb + UNR - 1) / UNR * UNR;
+        for (int k0 = kb; k0 < kend; k0 += UNR) {
+#pragma unroll
+            for (int jj = 0; jj < UNR; jj++) {
+                const int j = jj % D;
+                const int k = k0 + jj;
+                wait_vmcnt<(D - 1) * PAIR>();  // P(k): A(k) and B(k)
+                stage_barrier();               // A(k) visible; slot of step k - 1 free
+                const i32x4 *cur = ring + (size_t)(((k - kb) % S) * BMP + wm * TM) * 64 + lane;
+                i32x4 af[2];
+                af[0] = cur[0];
+#pragma unroll
+                for (int m = 0; m < TM; m++) {
+                    if (m + 1 < TM) af[(m + 1) & 1] = cur[(m + 1) * 64];
+#pragma unroll
+                    for (int n = 0; n < TN; n++)
+                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[m & 1], breg[j][n], acc[m][n], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                issue_a(k + D);  // into the slot of step k - 1
+                load_b(k + D, breg[j]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        wait_vmcnt<0>();  // loads past the range land before the workgroup's LDS is released
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int rt = mt0 + wm * TM + m;
+        if (rt >= MT) continue;
+        if (PACK) {
+#pragma unroll
+            for (int n = 0; n < TN; n += 4) {
+                const int t = bt0 + n;
+                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < TN; n++) {
+                const int t = bt0 + n;
+                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+            }
+        }
+    }
+}
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+__global__ __launch_bounds__(64 * WM * WN, 1) void dmac_kernel(const i32x4 *__restrict__ a_src,
+                                                               const i32x4 *__restrict__ b_src, GemmArgs g) {
+    dmac_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a_src, b_src, g, blockIdx.x);
+}
+
 // ---------------------------------------------------- both contractions in one launch --
 
 // Tile shapes as types, so that one kernel can run the tiles of two contractions.
@@ -1184,6 +1299,15 @@ struct DmabShape {
     static constexpr size_t kLds = (size_t)(D + 2) * ((WM * TM + 7) / 8 * 8) * kBlockBytes;
     static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
         dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a, b, g, L);
+    }
+};
+
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+struct DmacShape {
+    static constexpr int kThreads = 64 * WM * WN;
+    static constexpr size_t kLds = (size_t)(D + 1) * ((WM * TM + 7) / 8 * 8) * kBlockBytes;
+    static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
+        dmac_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a, b, g, L);
     }
 };
 
