@@ -5,6 +5,8 @@
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -483,6 +485,121 @@ int bucket_of(int MT) {
 
 struct Context;
 
+// Host threads for the copies of the synchronous entry point (as many as the `threads` the
+// caller gives the engine, at most 16): run(n, f) calls f(0) .. f(n - 1) on the workers and the
+// calling thread and returns when all are done.
+class HostPool {
+  public:
+    explicit HostPool(int threads) {
+        for (int i = 1; i < threads; i++) workers_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+    void run(size_t n, const std::function<void(size_t)> &f) {
+        if (workers_.empty() || n < 2) {
+            for (size_t i = 0; i < n; i++) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &f;
+            n_ = n;
+            next_ = 0;
+            active_ = (int)workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return active_ == 0; });
+    }
+
+  private:
+    void work() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+    }
+    void loop() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    int active_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+};
+
+// Host side of the synchronous entry point: a ring of pinned buffers through which the host
+// threads stage the frames, a chunk at a time (the DMA copy of one chunk runs while the next is
+// staged), and mapped pinned memory that the tail writes the responses into.
+struct HostStage {
+    static constexpr int kBufs = 4;
+    HostPool pool;
+    size_t buf_bytes = 0;
+    uint8_t *buf[kBufs] = {};
+    hipEvent_t copied[kBufs] = {};  // recorded after the DMA copy that reads buf[i]
+    bool pending[kBufs] = {};
+    uint8_t *resp = nullptr, *resp_dev = nullptr;
+    size_t resp_bytes = 0;
+    explicit HostStage(int threads) : pool(threads) {}
+    ~HostStage() {
+        for (int i = 0; i < kBufs; i++) {
+            if (pending[i]) (void)hipEventSynchronize(copied[i]);
+            if (copied[i]) (void)hipEventDestroy(copied[i]);
+            if (buf[i]) (void)hipHostFree(buf[i]);
+        }
+        if (resp) (void)hipHostFree(resp);
+    }
+    bool reserve(size_t chunk, size_t rbytes) {
+        bool ok = true;
+        if (chunk > buf_bytes) {
+            for (int i = 0; i < kBufs; i++) {
+                if (pending[i]) (void)hipEventSynchronize(copied[i]);
+                pending[i] = false;
+                if (buf[i]) (void)hipHostFree(buf[i]);
+                buf[i] = nullptr;
+            }
+            buf_bytes = 0;
+            for (int i = 0; ok && i < kBufs; i++) {
+                ok = HIP_OK(hipHostMalloc((void **)&buf[i], chunk, hipHostMallocDefault));
+                if (ok && !copied[i]) ok = HIP_OK(hipEventCreateWithFlags(&copied[i], hipEventDisableTiming));
+            }
+            if (!ok) return false;
+            buf_bytes = chunk;
+        }
+        if (rbytes > resp_bytes) {
+            if (resp) (void)hipHostFree(resp);
+            resp = resp_dev = nullptr;
+            resp_bytes = 0;
+            ok = HIP_OK(hipHostMalloc((void **)&resp, rbytes, hipHostMallocMapped)) &&
+                 HIP_OK(hipHostGetDevicePointer((void **)&resp_dev, resp, 0));
+            if (!ok) return false;
+            resp_bytes = rbytes;
+        }
+        return true;
+    }
+};
+
 struct Engine {
     int device = 0;
     size_t ld = 0, rows = 0, cols = 0, n = 0, tg = 0, kp = 0, outputs = 0;
@@ -504,6 +621,7 @@ struct Engine {
     int fuse = -1;               // one launch for scan and packing: -1 built-in, 0 never, 1 every pair
     bool time_graphs = false;    // graph replays record the stage events too
     Context *ctx0 = nullptr;     // the context of alto_answer / alto_answer_device
+    HostStage *host = nullptr;   // host staging of alto_answer (created on its first call)
 
     Engine() {
         for (auto &t : tuned)
@@ -729,6 +847,7 @@ struct Context {
 Engine::~Engine() {
     (void)hipSetDevice(device);
     delete ctx0;
+    delete host;
     for (void *p : {(void *)db, (void *)u, (void *)ucorr, (void *)that, (void *)wtab, (void *)prefix})
         if (p) (void)hipFree(p);
 }
@@ -1628,10 +1747,29 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
     for (size_t c = 0; c < B; c++) ctx->hstatus[c] = (!req[c] || len[c] != want) ? 1 : 0;
     hipStream_t st = ctx->stream;
     const FrameSet fs{ctx->slots, S, nullptr};
+    if (!e->host) e->host = new HostStage(std::max(1, std::min(e->threads, 16)));
+    HostStage &h = *e->host;
+    // frames: staged by the host threads into pinned buffers, about 4 MiB of slots at a time
+    // (pieces of up to 1 MiB per thread), each chunk's DMA copy overlapping the next chunk's
+    // staging; a pinned buffer is reused once its copy has completed
+    constexpr size_t kPiece = (size_t)1 << 20;
+    const size_t per = std::max<size_t>(1, ((size_t)4 << 20) / S), pieces = (want + kPiece - 1) / kPiece;
+    if (!h.reserve(per * S, B * rl)) return -1;
     bool ok = HIP_OK(hipEventRecord(ctx->next_set()[0], st));
-    for (size_t c = 0; ok && c < B; c++)
-        if (!ctx->hstatus[c])
-            ok = HIP_OK(hipMemcpyAsync(ctx->slots + c * S + ALTO_FRAME_OFFSET, req[c], want, hipMemcpyHostToDevice, st));
+    for (size_t c0 = 0, ci = 0; ok && c0 < B; c0 += per, ci++) {
+        const size_t c1 = std::min(B, c0 + per);
+        const int b = (int)(ci % HostStage::kBufs);
+        if (h.pending[b]) ok = HIP_OK(hipEventSynchronize(h.copied[b]));
+        h.pending[b] = false;
+        uint8_t *dst = h.buf[b];
+        h.pool.run((c1 - c0) * pieces, [&](size_t t) {
+            const size_t c = c0 + t / pieces, off = (t % pieces) * kPiece;
+            if (!ctx->hstatus[c]) memcpy(dst + (c - c0) * S + ALTO_FRAME_OFFSET + off, req[c] + off, std::min(kPiece, want - off));
+        });
+        ok = ok && HIP_OK(hipMemcpyAsync(ctx->slots + c0 * S, dst, (c1 - c0) * S, hipMemcpyHostToDevice, st)) &&
+             HIP_OK(hipEventRecord(h.copied[b], st));
+        h.pending[b] = ok;
+    }
     ok = ok && HIP_OK(hipMemcpyAsync(ctx->status, ctx->hstatus.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
     if (!ok) return -1;
     if (!e->tail_ok) {
@@ -1645,19 +1783,19 @@ int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B,
             if (status[c] == 0) return -3;
         return 0;
     }
-    // the upload is timed as part of the validation stage (ev[0] was recorded before it)
-    if (ctx->enqueue(ctx->ws(), fs, B, ctx->out, rl, nullptr, ctx->status, st, true, false)) return -1;
+    // the upload is timed as part of the validation stage (ev[0] was recorded before it); the
+    // tail writes the responses into the mapped pinned buffer, from which the host threads copy
+    // them to the caller's buffers
+    if (ctx->enqueue(ctx->ws(), fs, B, h.resp_dev, rl, nullptr, ctx->status, st, true, false)) return -1;
     if (!HIP_OK(hipMemcpyAsync(status, ctx->status, B * sizeof(int), hipMemcpyDeviceToHost, st)) ||
         !HIP_OK(hipStreamSynchronize(st)))
         return -1;
+    for (int b = 0; b < HostStage::kBufs; b++) h.pending[b] = false;  // the stream has drained
     int answered = 0;
-    for (size_t c = 0; c < B; c++)
-        if (status[c] == 0) {
-            answered++;
-            if (!HIP_OK(hipMemcpyAsync(out[c], ctx->out + c * rl, rl, hipMemcpyDeviceToHost, st))) return -1;
-        }
-    // the download is timed as part of the tail stage
-    if (!HIP_OK(hipEventRecord(ctx->ev[6], st)) || !HIP_OK(hipStreamSynchronize(st))) return -1;
+    for (size_t c = 0; c < B; c++) answered += status[c] == 0;
+    h.pool.run(B, [&](size_t c) {
+        if (status[c] == 0) memcpy(out[c], h.resp + c * rl, rl);
+    });
     if (stage_ms) {
         // {validate, scan, pack, tail} as in src/amx_kernels.cpp:613, from the six stages
         double t[6];

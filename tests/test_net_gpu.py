@@ -5,6 +5,7 @@ stream; 9 when the slot pool is full) and, for a valid frame, the oracle's respo
 from __future__ import annotations
 
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -28,6 +29,17 @@ def oracle_responses(inp, frames):
     out = np.zeros((len(frames), inp.response_len), np.uint8)
     oc.oracle_answer(inp, frames, out, np.full(len(frames), -1, np.int32))
     return out
+
+
+def settled_stats(ns, key, value, timeout_s=5.0):
+    # the front end counts a reply once its write has returned, which can be after the client
+    # has read it
+    deadline = time.monotonic() + timeout_s
+    st = ns.stats()
+    while st[key] != value and time.monotonic() < deadline:
+        time.sleep(0.005)
+        st = ns.stats()
+    return st
 
 
 def read_all(s, count):
@@ -68,7 +80,7 @@ def test_frames_over_tcp_get_the_oracle_answer_and_the_decoder_status(hip_device
             assert body == want[rid].tobytes(), f"id {rid}: response differs from the oracle"
         else:
             assert body == b""
-    st = ns.stats()
+    st = settled_stats(ns, "answered", 3)
     assert st["frames"] == len(frames) and st["discarded"] == 1 and st["answered"] == 3
     ns.close()
     srv.close()
@@ -136,7 +148,7 @@ def test_two_databases_on_two_ports_and_a_stop_frame(hip_device, dbs):
     net.send_stop(s)
     assert ns.wait(5000), "the stop frame was not seen"
     s.close()
-    st = ns.stats()
+    st = settled_stats(ns, "answered", 72)
     assert st["connections"] == 7 and st["answered"] == 72 and st["stop_frames"] == 1
     ns.close()
     srv.close()
