@@ -17,7 +17,7 @@ SYMBOLS = (
     "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
     "alto_answer_device", "alto_answer_device_noalloc", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
-    "alto_engine_new_device", "alto_engine_new_records", "alto_context_new", "alto_context_free", "alto_engine_context", "alto_context_reserve",
+    "alto_engine_new_device", "alto_engine_new_records", "alto_context_new", "alto_context_bytes", "alto_context_free", "alto_engine_context", "alto_context_reserve",
     "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
     "alto_context_set_graphs", "alto_context_graphs", "alto_set_fusion", "alto_fused", "alto_set_graph_timing",
     "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
@@ -97,6 +97,8 @@ def load() -> C.CDLL:
                                             C.c_uint64, C.c_int]
     lib.alto_context_new.restype = P
     lib.alto_context_new.argtypes = [P]
+    lib.alto_context_bytes.restype = SZ
+    lib.alto_context_bytes.argtypes = [P, SZ]
     lib.alto_context_free.restype = None
     lib.alto_context_free.argtypes = [P]
     lib.alto_engine_context.restype = P

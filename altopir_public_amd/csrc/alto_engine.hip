@@ -1640,6 +1640,12 @@ int alto_answer_device(void *p, const uint8_t *slots, size_t slot_stride, size_t
     return alto_answer_gather(e->ctx0, slots, slot_stride, nullptr, B, out, out_stride, nullptr, status, stream);
 }
 
+size_t alto_context_bytes(const void *p, size_t B) {
+    auto *e = (const Engine *)p;
+    if (!e) return 0;
+    return workspace_bytes(*e, round_up(std::max<size_t>(B, 1), 4)) + 2 * sizeof(int);
+}
+
 int alto_answer_device_noalloc(void *p, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
                                size_t out_stride, int *status, void *stream) {
     auto *e = (Engine *)p;

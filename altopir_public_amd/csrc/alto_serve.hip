@@ -19,8 +19,10 @@
 //   complete a completion thread per context waits for that context's batch, stamps the
 //            completion time and hands the tickets to poll(), so batches complete in the
 //            order they end, whatever database they belong to.
-// Batches of one database overlap on the device as far as its contexts allow; the number
-// of contexts is bounded by the device memory free at start (memory admission).  Frames
+// Batches of one database overlap on the device as far as its contexts allow.  A database
+// starts with one context; the dispatcher admits another (up to contexts_per_db) when
+// requests wait, every context is busy and the device has the memory for one more context's
+// workspace with as much again to spare (memory admission at run time).  Frames
 // whose length is wrong get status 1 at once, and status 9 when every slot is taken
 // (src/bin/altopir.rs:417-436); the device validates the rest (statuses 2 and 3).
 #include <hip/hip_runtime.h>
@@ -129,7 +131,13 @@ struct Db {
     // markers can sit in a hardware queue shared with a context's stream behind that batch's
     // kernels (a waited-for upload stream returned only when the batch in flight ended)
     std::vector<hsa_signal_t> up_sigs;
-    std::vector<Ctx> ctx;
+    // contexts (a deque: the completer of each keeps a reference to its element while more
+    // are admitted); at most max_ctx, each needing about ctx_bytes of device memory
+    std::deque<Ctx> ctx;
+    int max_ctx = 1;
+    size_t ctx_bytes = 0;
+    bool admitting = false;            // the dispatcher is creating a context
+    long long refused_at_batch = -1;   // admission refused when this many batches had run
 };
 
 struct Server {
@@ -143,6 +151,8 @@ struct Server {
     std::string err;
     alto_server_stats stats{};
     int device = 0;
+    int max_batch_ctx = 0;  // queries each context is reserved for
+    int graph_mode = kNoGraphChange;  // the last alto_server_set_graphs mode, for new contexts
     hsa_agent_t gpu_agent{}, cpu_agent{};  // the device and a host agent, for the DMA copies
 };
 
@@ -267,12 +277,40 @@ bool dma_runs(Server *s, Db &d, std::vector<int> &runs) {
     return good;
 }
 
+// One context of database d, reserved for max_batch queries: its engine context, stream,
+// completion event and index/status buffers.  On failure frees what it made.
+bool make_context(Server *s, Db &d, Ctx &c) {
+    const int mb = s->max_batch_ctx;
+    c = Ctx{};
+    c.ctx = alto_context_new(d.engine);
+    bool good = c.ctx && alto_context_reserve(c.ctx, (size_t)mb) == 0;
+    good = good && hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&c.done, hipEventDisableTiming) == hipSuccess &&
+           hipMalloc(&c.d_idx, mb * sizeof(int)) == hipSuccess &&
+           hipHostMalloc((void **)&c.h_idx, mb * sizeof(int), hipHostMallocDefault) == hipSuccess &&
+           hipHostMalloc((void **)&c.h_status, mb * sizeof(int), hipHostMallocMapped) == hipSuccess &&
+           hipHostGetDevicePointer((void **)&c.d_status, c.h_status, 0) == hipSuccess;
+    if (!good) {
+        if (c.ctx) alto_context_free(c.ctx);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+        if (c.done) (void)hipEventDestroy(c.done);
+        if (c.d_idx) (void)hipFree(c.d_idx);
+        if (c.h_idx) (void)hipHostFree(c.h_idx);
+        if (c.h_status) (void)hipHostFree(c.h_status);
+        c = Ctx{};
+    }
+    return good;
+}
+
+void completer(Server *s, int di, int ci);
+
 void dispatcher(Server *s, int di) {
     (void)hipSetDevice(s->device);
     Db &d = *s->dbs[di];
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         int ci = -1;
+        bool admit = false;
         s->cv_dispatch.wait(lk, [&] {
             if (s->stop) return true;
             if (d.pending.empty()) return false;
@@ -281,9 +319,37 @@ void dispatcher(Server *s, int di) {
                     ci = (int)i;
                     return true;
                 }
+            // every context busy: try one more, unless the last attempt was refused and no
+            // batch has completed since
+            if ((int)d.ctx.size() < d.max_ctx && !d.admitting && d.refused_at_batch != s->stats.batches) {
+                admit = true;
+                return true;
+            }
             return false;
         });
         if (s->stop) return;
+        if (admit) {
+            // memory admission: a context only if the device then still has its size free
+            d.admitting = true;
+            const size_t need = d.ctx_bytes;
+            lk.unlock();
+            size_t free_b = 0, total_b = 0;
+            (void)hipMemGetInfo(&free_b, &total_b);
+            Ctx c;
+            const bool made = free_b >= 2 * need && make_context(s, d, c);
+            lk.lock();
+            d.admitting = false;
+            if (!made) {
+                d.refused_at_batch = s->stats.batches;
+                s->stats.contexts_refused++;
+                continue;
+            }
+            c.graph_req = s->graph_mode;
+            d.ctx.push_back(c);
+            s->stats.contexts_admitted++;
+            if (!s->stop) s->threads.emplace_back(completer, s, di, (int)d.ctx.size() - 1);
+            continue;  // the new context is free: it takes the waiting requests next
+        }
         Ctx &c = d.ctx[ci];
         c.batch.clear();
         while (!d.pending.empty() && (int)c.batch.size() < s->max_batch) {
@@ -433,6 +499,7 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
     if (!engines || n_engines <= 0 || max_batch <= 0 || slots_per_db <= 0 || contexts_per_db <= 0) return nullptr;
     auto *s = new Server;
     s->max_batch = max_batch;
+    s->max_batch_ctx = max_batch;
     (void)hipGetDevice(&s->device);
     for (int i = 0; i < n_engines; i++) s->dbs.emplace_back(new Db);
     bool good = find_agents(s);
@@ -466,34 +533,15 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
         d.leased.assign(d.nslots, 0);
         d.t_submit.resize(d.nslots);
         for (int k = 0; k < d.nslots; k++) d.free_slots.push_back(k);
-        // memory admission: as many contexts (each reserved for max_batch queries) as fit in
-        // 90% of the device memory free now, at least one, at most contexts_per_db
-        for (int k = 0; good && k < contexts_per_db; k++) {
-            size_t free_b = 0, total_b = 0;
-            (void)hipMemGetInfo(&free_b, &total_b);
+        // one context to start with (more are admitted at run time, up to contexts_per_db,
+        // each by its device footprint)
+        d.max_ctx = contexts_per_db;
+        d.ctx_bytes = alto_context_bytes(d.engine, (size_t)max_batch) + 2 * (size_t)max_batch * sizeof(int);
+        if (good) {
             Ctx c;
-            c.ctx = alto_context_new(d.engine);
-            if (!c.ctx) {
-                good = k > 0;
-                break;
-            }
-            if (alto_context_reserve(c.ctx, max_batch) != 0) {
-                alto_context_free(c.ctx);
-                good = k > 0;
-                if (!good) s->err = "not enough device memory for one batch of max_batch queries";
-                break;
-            }
-            size_t free_after = 0;
-            (void)hipMemGetInfo(&free_after, &total_b);
-            good = ok(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), s, "stream") &&
-                   ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), s, "event") &&
-                   ok(hipMalloc(&c.d_idx, max_batch * sizeof(int)), s, "index") &&
-                   ok(hipHostMalloc((void **)&c.h_idx, max_batch * sizeof(int), hipHostMallocDefault), s, "index") &&
-                   ok(hipHostMalloc((void **)&c.h_status, max_batch * sizeof(int), hipHostMallocMapped), s, "status") &&
-                   ok(hipHostGetDevicePointer((void **)&c.d_status, c.h_status, 0), s, "status mapping");
-            d.ctx.push_back(c);
-            const size_t used = free_b > free_after ? free_b - free_after : 0;
-            if (free_after < used + free_b / 10) break;
+            good = make_context(s, d, c);
+            if (!good) s->err = "not enough device memory for one batch of max_batch queries";
+            else d.ctx.push_back(c);
         }
     }
     if (!good) {
@@ -502,10 +550,13 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
         fprintf(stderr, "alto_server_new: %s\n", e.c_str());
         return nullptr;
     }
-    for (int i = 0; i < n_engines; i++) {
-        s->threads.emplace_back(dispatcher, s, i);
-        s->threads.emplace_back(uploader, s, i);
-        for (int k = 0; k < (int)s->dbs[i]->ctx.size(); k++) s->threads.emplace_back(completer, s, i, k);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);  // dispatchers add completers under the lock
+        for (int i = 0; i < n_engines; i++) {
+            s->threads.emplace_back(uploader, s, i);
+            for (int k = 0; k < (int)s->dbs[i]->ctx.size(); k++) s->threads.emplace_back(completer, s, i, k);
+            s->threads.emplace_back(dispatcher, s, i);
+        }
     }
     return s;
 }
@@ -513,6 +564,7 @@ void *alto_server_new(void *const *engines, int n_engines, int max_batch, int sl
 int alto_server_contexts(void *p, int db) {
     auto *s = (Server *)p;
     if (!s || db < 0 || db >= (int)s->dbs.size()) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
     return (int)s->dbs[db]->ctx.size();
 }
 
@@ -613,7 +665,9 @@ void alto_server_free(void *p) {
     s->cv_done.notify_all();
     s->cv_complete.notify_all();
     for (auto &dp : s->dbs) dp->cv_upload.notify_all();
-    for (auto &t : s->threads) t.join();
+    // no thread is added once stop is set (the dispatchers add completers under the lock
+    // only while it is clear), so the list is fixed while it is joined
+    for (size_t i = 0; i < s->threads.size(); i++) s->threads[i].join();
     for (auto &dp : s->dbs) {
         Db &d = *dp;
         for (auto &c : d.ctx) {
@@ -659,6 +713,7 @@ int alto_server_set_graphs(void *p, int mode) {
     // recorded here and applied by each context's dispatcher before its next batch: a busy
     // context may be inside alto_answer_gather (capturing or launching a graph) right now
     std::lock_guard<std::mutex> lk(s->mu);
+    s->graph_mode = mode;  // for contexts admitted later
     for (auto &dp : s->dbs)
         for (auto &c : dp->ctx) c.graph_req = mode;
     return 0;

@@ -25,7 +25,8 @@ class Completion(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("submitted", C.c_longlong), ("queue_full", C.c_longlong), ("batches", C.c_longlong),
-                ("answered_in_batches", C.c_longlong), ("max_batch_seen", C.c_longlong)]
+                ("answered_in_batches", C.c_longlong), ("max_batch_seen", C.c_longlong),
+                ("contexts_admitted", C.c_longlong), ("contexts_refused", C.c_longlong)]
 
 
 class ReplayResult(C.Structure):

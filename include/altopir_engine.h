@@ -163,6 +163,9 @@ int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t
 int alto_answer_device_noalloc(void *e, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
                                size_t out_stride, int *status, void *stream);
 
+/* Device bytes a context of engine e holds once reserved for B queries (its per-batch
+ * workspace; the serving loop admits contexts by it).  No counterpart in the reference. */
+size_t alto_context_bytes(const void *e, size_t B);
 void *alto_context_new(void *e);
 void alto_context_free(void *ctx);
 void *alto_engine_context(void *e);
@@ -269,9 +272,11 @@ int alto_tune_get(const void *e, int which, size_t B);
  * links libhsa-runtime64 and holds one reference on it for the life of the process), and
  * whenever one of a database's contexts is free, every waiting request of that database
  * (up to max_batch) is answered as one batch; batches of one database overlap on the
- * device as far as its contexts allow.  The number of contexts per database is at most
- * contexts_per_db and at most what fits in the device memory free at creation (each is
- * reserved for max_batch queries).  Responses are written by the device straight into
+ * device as far as its contexts allow.  Each context is reserved for max_batch queries; a
+ * database starts with one and gets another, up to contexts_per_db, when requests wait, all
+ * of its contexts are busy and the device memory then free is at least twice one context's
+ * footprint (memory admission at run time; a refused admission is retried after the next
+ * batch completes; alto_server_get_stats counts both).  Responses are written by the device straight into
  * pinned host memory and stay valid until alto_server_release.  Creating a server writes
  * every page of its pinned slot pools once (slots_per_db x frame size per database), so
  * that no submit takes first-touch page faults. */
@@ -290,6 +295,7 @@ typedef struct {
 
 typedef struct {
     long long submitted, queue_full, batches, answered_in_batches, max_batch_seen;
+    long long contexts_admitted, contexts_refused;  /* run-time memory admission (beyond the first) */
 } alto_server_stats;
 
 void *alto_server_new(void *const *engines, int n_engines, int max_batch, int slots_per_db,

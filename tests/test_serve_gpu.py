@@ -169,6 +169,54 @@ def test_saturated_loop_answers_everything_in_full_batches(hip_device, two_dbs):
     eng.close()
 
 
+def test_contexts_are_admitted_at_run_time_by_free_memory(hip_device, two_dbs):
+    # a database starts with one context and gets more (up to contexts_per_db) only while
+    # requests wait behind busy contexts and the device has the memory; with the device nearly
+    # full the one context answers everything and the refusals are counted, and once memory
+    # is free again the next load admits another context
+    import torch
+
+    inp = two_dbs[0]
+    eng = Engine.from_inputs(inp)
+    frames = [np.ascontiguousarray(f) for f in workload.make_requests(16, inp.rows, inp.kp, seed=88)]
+    srv = Server([eng], max_batch=32, slots=160, contexts=3)
+    assert srv.contexts(0) == 1
+    nat = srv.replay(frames, total=960, threads=8, window=160)
+    st = srv.stats()
+    assert nat["answered"] == 960 and nat["rejected"] == 0
+    assert srv.contexts(0) >= 2 and st["contexts_admitted"] == srv.contexts(0) - 1
+    srv.close()
+    # the device footprint of one context reserved for 512 queries; the device is then filled
+    # to leave 1.5 times that free (admission wants twice)
+    from altopir_public_amd import _lib
+
+    need = _lib.load().alto_context_bytes(eng.ptr, 512)
+    assert need > 32 << 20
+    srv = Server([eng], max_batch=512, slots=1536, contexts=2)
+    # filled in steps until the free memory the device reports (what admission reads, which
+    # does not drop by exactly what is allocated) is below 1.5 times the footprint
+    hog = []
+    for _ in range(64):
+        free, _ = torch.cuda.mem_get_info()
+        if free < 1.5 * need:
+            break
+        try:
+            hog.append(torch.empty(max(int(free - 1.2 * need), 4 << 20), dtype=torch.uint8, device="cuda"))
+        except RuntimeError:  # out of memory: the rest stays
+            break
+    assert torch.cuda.mem_get_info()[0] < 2 * need
+    nat = srv.replay(frames, total=3072, threads=8, window=1536)
+    st = srv.stats()
+    assert nat["answered"] == 3072 and nat["rejected"] == 0
+    assert srv.contexts(0) == 1 and st["contexts_refused"] >= 1 and st["contexts_admitted"] == 0
+    del hog
+    torch.cuda.empty_cache()
+    nat = srv.replay(frames, total=3072, threads=8, window=1536)
+    assert nat["answered"] == 3072 and srv.contexts(0) == 2 and srv.stats()["contexts_admitted"] == 1
+    srv.close()
+    eng.close()
+
+
 def test_load_generator_end_to_end(hip_device, capsys):
     # two small resident databases, Poisson arrivals with 5% corrupted frames: each corrupted
     # frame is rejected with the decoder's status, every valid one answered
