@@ -303,6 +303,10 @@ const Variant kVariants[] = {
     // the same with the A fragments read from LDS just before their MFMAs
     {"dmac8_m8_n4_d2u16", launch_dmac<8, 4, 2, 4, 2, 16, false>, launch_dmac<8, 4, 2, 4, 2, 16, true>,
      prepare_dmac<8, 4, 2, 4, 2, 16, false>, prepare_dmac<8, 4, 2, 4, 2, 16, true>},
+    // 12 x 16 tiles, one wave per column of two B tiles (every B block loaded once per
+    // workgroup), four k-steps of B loads in flight; scan only (packing on dmab8_m6_n4_u16)
+    {"dmac8_w18_m12_n2_d4u16", launch_dmac<12, 2, 1, 8, 4, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
+     prepare_dmac<12, 2, 1, 8, 4, 16, false>, prepare_dmab<6, 4, 2, 4, 2, 16, true>},
     // the same with 12 x 16 and 8 x 32-tile workgroups (the 8 x 32 one with one wave per
     // column of four B tiles), for 17-48 queries
     {"dmab8_m6_n4_u16", launch_dmab<6, 4, 2, 4, 2, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
@@ -386,29 +390,6 @@ struct FusedRegB {
     }
 };
 using FusedRegB16 = FusedRegB<8, 4, 2, 4, 2, 16>;
-// Scan on the just-in-time-fragment form, packing on the register-streaming 16 x 16 tiles.
-struct FusedRegC16 {
-    using SA = DmacShape<8, 4, 2, 4, 2, 16, false, true>;
-    using SN = DmacShape<8, 4, 2, 4, 2, 16, false, false>;
-    using P = DmabShape<8, 4, 2, 4, 2, 16, true, false>;
-    static constexpr size_t kLds = SA::kLds > P::kLds ? SA::kLds : P::kLds;
-    static void prepare() {
-        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SA, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kLds);
-        (void)hipFuncSetAttribute((const void *)scan_pack_kernel<SN, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kLds);
-    }
-    static void launch(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya,
-                       const int8_t *u, hipStream_t st) {
-        const int sb = plan_ring<8, 4, 2, 4>(gs, false), pb = plan_ring<8, 4, 2, 4>(gp, true);
-        if (gs.gz > 1)
-            hipLaunchKernelGGL((scan_pack_kernel<SA, P>), dim3(sb + pb), dim3(SA::kThreads), kLds, st,
-                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
-        else
-            hipLaunchKernelGGL((scan_pack_kernel<SN, P>), dim3(sb + pb), dim3(SN::kThreads), kLds, st,
-                               (const i32x4 *)qa, (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb);
-    }
-};
 using FusedRing8 = FusedRing<4, 8, 4, 2, 4, 1, 4, 8, 4, 2, 4, 1>;
 using FusedRingWide = FusedRing<8, 6, 2, 4, 4, 2, 4, 8, 4, 2, 4, 1>;
 
@@ -424,7 +405,6 @@ const FusedPair kFused[] = {
     {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
     {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
     {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", FusedRegB16::launch, FusedRegB16::prepare},
-    {"dmac8_m8_n4_d2u16", "dmab8_m8_n4_u16", FusedRegC16::launch, FusedRegC16::prepare},
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
@@ -439,22 +419,23 @@ constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 // contractions on the 16 x 16-tile workgroups that stream the B blocks into registers, 16
 // k-steps per loop iteration (at batch 128: scan 0.432 against 0.492 ms for the best all-LDS
 // ring at 1 GiB, 30.3 against 30.6 ms at 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB
-// batch 2048 59.2 against 60.1 ms).  A scan of more than 48 queries over a record range that
-// runs unsplit (KC <= kFoldSteps, at most 65,536 records) takes the form that reads its A
-// fragments just before their MFMAs: 3-10% faster at 1 GiB and 5-9% at 2 GiB (batches of 64 and
-// 128), while with two or more splits it was 1-5% slower (4 and 64 GiB at batch 128).
+// batch 2048 59.2 against 60.1 ms).  The scan of 33-48 queries over a record range that runs
+// unsplit (KC <= kFoldSteps, at most 65,536 records) takes 12 x 16 tiles that read their A
+// fragments just before their MFMAs with four k-steps of B loads in flight: 6-10% faster than
+// the 12 x 20 ring at 1 and 2 GiB, in either order of measurement; with the range split (4 GiB
+// and up) the ring stays ahead (8 GiB, 37 queries: 1.70 against 1.83 ms).
 int variant_index(const char *name);
 int default_variant(int which, int MT, int KC) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
                      rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
                      regb = variant_index("dmab8_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
-                     regc = variant_index("dmac8_m8_n4_d2u16");
+                     regc12 = variant_index("dmac8_w18_m12_n2_d4u16");
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
     if (MT <= 8) return rows8;
-    if (MT <= 12) return rows12;
-    return KC <= kFoldSteps ? regc : regb;
+    if (MT <= 12) return KC <= kFoldSteps ? regc12 : rows12;
+    return regb;
 }
 int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)
