@@ -85,7 +85,8 @@ def main(argv=None) -> int:
     pools = [frame_pool(inp, args.pool, seed=args.seed * 1000 + i) for i, inp in enumerate(inputs)]
     srv = serve.Server(engines, max_batch=args.max_batch, slots=args.slots, contexts=args.contexts)
     ctxs = [srv.contexts(i) for i in range(len(engines))]
-    log(f"[load] serving {len(engines)} databases, contexts per database {ctxs}")
+    log(f"[load] serving {len(engines)} databases, contexts per database {ctxs} to start with, "
+        f"up to {args.contexts} admitted under load")
     for k, rate in enumerate(args.rates):
         count = max(1, int(round(rate * args.duration)))
         rng = np.random.default_rng([args.seed, k])
@@ -107,7 +108,8 @@ def main(argv=None) -> int:
         res = serve.run_load(srv, frames, [int(x) for x in dbs], rate, seed=args.seed * 7919 + k, on_done=on_done)
         out = {"offered_rps": rate, "duration_s": args.duration, "databases_gib": args.gib, **res.summary(),
                "corrupted": int(bad.sum()), "rejected_as_expected": seen_bad == expect_bad,
-               "contexts": ctxs, "max_batch_limit": args.max_batch}
+               "contexts": [srv.contexts(i) for i in range(len(engines))],  # after this rate's load
+               "max_batch_limit": args.max_batch}
         print(json.dumps(out), flush=True)
         log(f"[load] {rate:g} req/s: {res.summary()}")
     srv.close()
