@@ -461,6 +461,15 @@ int bucket_of(int MT) {
     while (b + 1 < kBuckets && bucket_top(b) < MT) b++;
     return b;
 }
+constexpr int kMaxBucketTiles = 512;  // bucket graphs up to 2048 queries
+// Queries a workspace needs so that every batch of up to B queries runs without growing it:
+// B rounded up to the top of its bucket where bucket graphs are captured (those are sized for
+// the bucket's largest batch), else to a multiple of 4.
+size_t bucket_queries(size_t B) {
+    const size_t MT = (std::max<size_t>(B, 1) + 3) / 4;
+    if (MT > (size_t)kMaxBucketTiles) return 4 * MT;
+    return 4 * (size_t)bucket_top(bucket_of((int)MT));
+}
 
 // -------------------------------------------------------------------- engine --
 
@@ -712,7 +721,6 @@ struct Context {
     int graph_mode = -1;  // -1: the engine's setting, 0: eager, 1: capture and replay
     uint64_t tick = 0;
     static constexpr size_t kMaxGraphs = 64;
-    static constexpr int kMaxBucketTiles = 512;  // bucket graphs up to 2048 queries
     int *dims = nullptr;  // device {B, MT} of the batch a bucket graph answers
 
     bool init(Engine *eng) {
@@ -800,9 +808,14 @@ struct Context {
         const bool grow = Bp > cap;
         const bool add_host = host_arenas && !slots;
         if (!grow && !add_host) return true;
-        (void)hipStreamSynchronize(stream);
-        (void)hipDeviceSynchronize();  // a caller stream may still be using the buffers
-        drop_graphs();                 // they hold the old workspace addresses
+        if (cap || slots) {
+            // old buffers are freed below: a caller stream may still be using them.  A fresh
+            // context has none and synchronises nothing, so that admitting one while another
+            // thread captures a graph on its own stream cannot disturb that capture.
+            (void)hipStreamSynchronize(stream);
+            (void)hipDeviceSynchronize();
+        }
+        drop_graphs();  // they hold the old workspace addresses
         const size_t ncap = grow ? round_up(std::max(Bp, cap + cap / 2), 4) : cap;
         const bool with_host = host_arenas || slots;
         release_buffers();
@@ -1624,7 +1637,7 @@ int alto_answer_device(void *p, const uint8_t *slots, size_t slot_stride, size_t
 size_t alto_context_bytes(const void *p, size_t B) {
     auto *e = (const Engine *)p;
     if (!e) return 0;
-    return workspace_bytes(*e, round_up(std::max<size_t>(B, 1), 4)) + 2 * sizeof(int);
+    return workspace_bytes(*e, bucket_queries(B)) + 2 * sizeof(int);
 }
 
 int alto_answer_device_noalloc(void *p, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
@@ -1693,7 +1706,7 @@ int alto_context_reserve(void *c, size_t B) {
     auto *ctx = (Context *)c;
     if (!ctx) return -1;
     if (!HIP_OK(hipSetDevice(ctx->e->device))) return -1;
-    return ctx->reserve(B, false) ? 0 : -4;
+    return ctx->reserve(bucket_queries(B), false) ? 0 : -4;
 }
 
 int alto_answer_gather(void *c, const uint8_t *slots, size_t slot_stride, const int *slot_index, size_t B,

@@ -158,6 +158,10 @@ class Server:
         out.update(requests=total, threads=threads, window=window)
         return out
 
+    def error(self) -> str:
+        """The last error the serving loop recorded (alto_server_error), '' if none."""
+        return (self.lib.alto_server_error(self.ptr) or b"").decode(errors="replace")
+
     def stats(self) -> dict:
         s = Stats()
         self.lib.alto_server_get_stats(self.ptr, C.byref(s))

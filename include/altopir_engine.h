@@ -144,7 +144,9 @@ int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t
  * alto_answer and alto_answer_device use the engine's default context
  * (alto_engine_context); more contexts let several batches of one engine be in flight at
  * once (one batch per context at a time).  alto_context_reserve sizes a context's
- * workspace for B queries ahead of time (0, or -4 when device memory is short). */
+ * workspace ahead of time so that no batch of up to B queries grows it (B rounded up to the
+ * top of its batch-size bucket, whose captured graph is sized for that bucket's largest
+ * batch); 0, or -4 when device memory is short. */
 /* No counterpart in the reference: alto_answer_device for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, statuses, returns and results as alto_answer_device, but it
@@ -164,7 +166,8 @@ int alto_answer_device_noalloc(void *e, const uint8_t *slots, size_t slot_stride
                                size_t out_stride, int *status, void *stream);
 
 /* Device bytes a context of engine e holds once reserved for B queries (its per-batch
- * workspace; the serving loop admits contexts by it).  No counterpart in the reference. */
+ * workspace, sized as alto_context_reserve sizes it; the serving loop admits contexts by
+ * it).  No counterpart in the reference. */
 size_t alto_context_bytes(const void *e, size_t B);
 void *alto_context_new(void *e);
 void alto_context_free(void *ctx);

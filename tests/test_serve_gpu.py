@@ -87,7 +87,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix(hip_device, two_dbs, uploa
         t.join()
     assert max(x.batch for x in mix.values()) > 1, "the mix never shared a batch"
     for i in range(len(frames)):
-        assert alone[i].status == mix[i].status == want_status[i], i
+        assert alone[i].status == mix[i].status == want_status[i], (i, srv.error())
         if want_status[i] == 0:
             assert alone[i].response == want[i], f"request {i}: alone differs from the oracle"
             assert mix[i].response == want[i], f"request {i}: in the mix differs from the oracle"
@@ -118,7 +118,7 @@ def test_reused_slots_answer_each_new_frame(hip_device, two_dbs, upload):
             got[x.tag] = x
             outstanding -= 1
     for i in range(total):
-        assert got[i].status == 0, i
+        assert got[i].status == 0, (i, srv.error())
         assert got[i].response == want[i % 12].tobytes(), f"request {i} (frame {i % 12}) differs from the oracle"
     assert max(x.batch for x in got.values()) > 1
     srv.close()
