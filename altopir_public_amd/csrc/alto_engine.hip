@@ -408,6 +408,81 @@ const FusedPair kFused[] = {
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
+// Scan, packing and tail in one launch (step_kernel) for the shape pairs of the built-in
+// policy at every batch size (the pairs need one workgroup size).  PlanS / PlanP size the
+// two grids, BTS / BTP are the B tiles of one x tile of each (a scan B tile is 16 columns,
+// a packing one a quarter of that: four byte planes per column tile).
+typedef int (*StepFn)(GemmArgs, const int8_t *, const int8_t *, GemmArgs, const int8_t *, const int8_t *,
+                      const TailArgs &, StepArgs, hipStream_t);
+template <class SA, class SN, class P, int (*PlanS)(GemmArgs &, bool), int (*PlanP)(GemmArgs &, bool), int BTS,
+          int BTP>
+struct StepPairT {
+    static constexpr int kH = SA::kThreads / kTailThreads;
+    static constexpr size_t kTileLds = SA::kLds > P::kLds ? SA::kLds : P::kLds;
+    static void prepare() {
+        (void)hipFuncSetAttribute((const void *)step_kernel<SA, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kMaxLds);
+        (void)hipFuncSetAttribute((const void *)step_kernel<SN, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kMaxLds);
+    }
+    static size_t lds(int n, int part_words) {
+        return std::max(kTileLds, step_lds_bytes(kH, n, part_words));
+    }
+    static int launch(GemmArgs gs, const int8_t *qa, const int8_t *db, GemmArgs gp, const int8_t *ya, const int8_t *u,
+                      const TailArgs &ta, StepArgs sa, hipStream_t st) {
+        const size_t l = lds(ta.n, ta.part_words);
+        if (l > kMaxLds) return -1;
+        const int sb = PlanS(gs, false), pb = PlanP(gp, true);
+        sa.scan_cols = 16 * BTS;
+        sa.pack_cols = 4 * BTP;
+        if (gs.gz > 1)
+            hipLaunchKernelGGL((step_kernel<SA, P>), dim3(sb + pb), dim3(SA::kThreads), l, st, (const i32x4 *)qa,
+                               (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb, ta, sa);
+        else
+            hipLaunchKernelGGL((step_kernel<SN, P>), dim3(sb + pb), dim3(SN::kThreads), l, st, (const i32x4 *)qa,
+                               (const i32x4 *)db, gs, (const i32x4 *)ya, (const i32x4 *)u, gp, sb, ta, sa);
+        return 0;
+    }
+};
+template <int MB, int NBT, int U, int MBp, int NBTp, int Up>
+using StepStream = StepPairT<StreamShape<MB, NBT, U, false, true>, StreamShape<MB, NBT, U, false, false>,
+                             StreamShape<MBp, NBTp, Up, true, false>, plan_stream<MB, NBT>, plan_stream<MBp, NBTp>,
+                             4 * NBT, 4 * NBTp>;
+using StepRows8 = StepPairT<DmaShape<4, 8, 2, 4, 4, false, true, 1>, DmaShape<4, 8, 2, 4, 4, false, false, 1>,
+                            DmaShape<2, 8, 4, 2, 4, true, false, 1>, plan_ring<4, 8, 2, 4>, plan_ring<2, 8, 4, 2>, 32,
+                            16>;
+using StepRows12c = StepPairT<DmacShape<12, 2, 1, 8, 4, 16, false, true>, DmacShape<12, 2, 1, 8, 4, 16, false, false>,
+                              DmabShape<6, 4, 2, 4, 2, 16, true, false>, plan_ring<12, 2, 1, 8>,
+                              plan_ring<6, 4, 2, 4>, 16, 16>;
+using StepRows12 = StepPairT<DmaShape<3, 10, 4, 2, 4, false, true, 1>, DmaShape<3, 10, 4, 2, 4, false, false, 1>,
+                             DmabShape<6, 4, 2, 4, 2, 16, true, false>, plan_ring<3, 10, 4, 2>, plan_ring<6, 4, 2, 4>,
+                             20, 16>;
+using StepRegB16 = StepPairT<DmabShape<8, 4, 2, 4, 2, 16, false, true>, DmabShape<8, 4, 2, 4, 2, 16, false, false>,
+                             DmabShape<8, 4, 2, 4, 2, 16, true, false>, plan_ring<8, 4, 2, 4>, plan_ring<8, 4, 2, 4>,
+                             16, 16>;
+struct StepPair {
+    const char *scan, *pack;
+    StepFn launch;
+    PrepareFn prepare;
+    size_t (*lds)(int, int);
+};
+const StepPair kStep[] = {
+    {"stream_m2_n4_u4", "stream_m1_n4_u4", StepStream<2, 4, 4, 1, 4, 4>::launch, StepStream<2, 4, 4, 1, 4, 4>::prepare,
+     StepStream<2, 4, 4, 1, 4, 4>::lds},
+    {"stream_m2_n4_u4", "stream_m2_n4_u4", StepStream<2, 4, 4, 2, 4, 4>::launch, StepStream<2, 4, 4, 2, 4, 4>::prepare,
+     StepStream<2, 4, 4, 2, 4, 4>::lds},
+    {"stream_m1_n4_u4", "stream_m1_n4_u4", StepStream<1, 4, 4, 1, 4, 4>::launch, StepStream<1, 4, 4, 1, 4, 4>::prepare,
+     StepStream<1, 4, 4, 1, 4, 4>::lds},
+    {"dma8_m4_n8_w24_s4", "dma8_m2_n8_s4", StepRows8::launch, StepRows8::prepare, StepRows8::lds},
+    {"dmac8_w18_m12_n2_d4u16", "dmab8_m6_n4_u16", StepRows12c::launch, StepRows12c::prepare, StepRows12c::lds},
+    {"dma8_m3_n10_s4", "dmab8_m6_n4_u16", StepRows12::launch, StepRows12::prepare, StepRows12::lds},
+    {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", StepRegB16::launch, StepRegB16::prepare, StepRegB16::lds},
+};
+constexpr int kNumStep = (int)(sizeof(kStep) / sizeof(kStep[0]));
+// Polls a helper workgroup of the step kernel makes at most while it waits for parts to be
+// published (about 0.3 ms each 1,000; the waits end long before in every correct run).
+constexpr int kStepSpin = 1 << 20;
+
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
 // from sweeps over batch sizes 1-2048 on 1-64 GiB databases: streaming kernels up to 8
 // queries, and for packing up to 16 (the scan of 1-4 queries runs the 8-query streaming
@@ -441,6 +516,13 @@ int variant_index(const char *name) {
     for (int i = 0; i < kNumVariants; i++)
         if (!strcmp(kVariants[i].name, name)) return i;
     return 0;
+}
+
+// Index into kStep of the pair (scan variant vs, packing variant vp), or -1.
+int step_index(int vs, int vp) {
+    for (int i = 0; i < kNumStep; i++)
+        if (variant_index(kStep[i].scan) == vs && variant_index(kStep[i].pack) == vp) return i;
+    return -1;
 }
 
 // Index into kFused of the pair (scan variant vs, packing variant vp), or -1.
@@ -608,7 +690,8 @@ struct Engine {
     uint32_t k128e = 0, ycorr0 = 0;
     int tuned[2][kBuckets];      // [scan, pack][bucket] -> variant, -1 = built-in policy
     bool use_graphs = true;
-    int fuse = -1;               // one launch for scan and packing: -1 built-in, 0 never, 1 every pair
+    int fuse = -1;               // -1 built-in, 0 never, 1 scan and packing in one launch for every pair,
+                                 // 2 scan, packing and tail in one launch (step pairs; else as 1)
     bool time_graphs = false;    // graph replays record the stage events too
     Context *ctx0 = nullptr;     // the context of alto_answer / alto_answer_device
     HostStage *host = nullptr;   // host staging of alto_answer (created on its first call)
@@ -630,8 +713,27 @@ struct Engine {
     // then interleave with a much longer scan), at two A tiles 8% slower, from 16 A tiles on
     // level.
     int fused_for(int MT) const {
-        if (fuse == 0 || (fuse < 0 && (MT != 1 || rows > 4 * kp))) return -1;
+        if (fuse == 0) return -1;
+        if (fuse == 2 || (fuse < 0 && step_builtin(MT))) {
+            const int k = step_for(MT);
+            if (k >= 0) return kNumFused + k;
+        }
+        if (fuse < 0 && (MT != 1 || rows > 4 * kp)) return -1;
         return fused_index(variant_for(0, MT), variant_for(1, MT));
+    }
+    // Built-in use of the one-launch step: never (measured per batch size; see step_for).
+    bool step_builtin(int) const { return false; }
+    // kStep index for a batch of MT A tiles, or -1: the pair of the chosen scan and packing
+    // variants; when the packing variant is the built-in one and has no step pair with the
+    // scan (its workgroup size differs), the first step pair of that scan variant.
+    int step_for(int MT) const {
+        const int vs = variant_for(0, MT), vp = variant_for(1, MT);
+        int k = step_index(vs, vp);
+        if (k < 0 && tuned[1][bucket_of(MT)] < 0)
+            for (int i = 0; i < kNumStep && k < 0; i++)
+                if (variant_index(kStep[i].scan) == vs) k = i;
+        if (k >= 0 && kStep[k].lds((int)n, (int)part_words) > kMaxLds) return -1;
+        return k;
     }
     size_t request_len() const { return 9 + 4 * (2 * kp + rows); }
     size_t response_len() const { return outputs * part_len; }
@@ -647,12 +749,15 @@ struct Workspace {
     uint32_t *beta;            // packing results, cap * C
     uint32_t *qsum;            // cap
     unsigned long long *ysum;  // cap
+    uint32_t *sync;            // 3 * outputs + 4 words: part arrivals and tail claims of the step kernel
 };
 
 // Bytes of the workspace of a batch of Bp queries (Bp a multiple of 4), each buffer starting
 // on a 256-byte boundary, and its layout inside one block of memory at `base`.
 size_t workspace_bytes(const Engine &e, size_t Bp);
 Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base);
+// Words of the step kernel's synchronisation area (StepArgs::sync).
+size_t sync_words(const Engine &e) { return 3 * e.outputs + 4; }
 
 // One batch in flight: its workspace, stream, timing events and captured graphs.  Batches
 // on different contexts of one engine may run concurrently; one context runs one batch at
@@ -666,6 +771,7 @@ struct Context {
     uint32_t *beta = nullptr;    // cap * C
     uint32_t *qsum = nullptr;    // cap
     unsigned long long *ysum = nullptr;
+    uint32_t *sync = nullptr;    // 3 * outputs + 4 (step kernel)
     // arenas of the synchronous entry point (allocated on first use)
     size_t slot_stride = 0;
     uint8_t *slots = nullptr, *out = nullptr;
@@ -691,7 +797,7 @@ struct Context {
     long long stage_count = 0;
     bool timed_valid = false;
     void record(hipEvent_t x, hipStream_t st) { (void)hipEventRecord(x, st); }
-    bool add_stage_events(hipGraph_t graph, bool fused, int set);
+    bool add_stage_events(hipGraph_t graph, int plan, int set);
     std::vector<int> hstatus;
     // captured batches, replayed when the same batch (pointers, size, stream, kernel
     // variants) comes again
@@ -780,12 +886,14 @@ struct Context {
         graphs.clear();
     }
     void release_buffers() {
-        for (void *p : {(void *)qa, (void *)ya, (void *)acc, (void *)beta, (void *)qsum, (void *)ysum, (void *)slots,
+        for (void *p : {(void *)qa, (void *)ya, (void *)acc, (void *)beta, (void *)qsum, (void *)ysum, (void *)sync,
+                        (void *)slots,
                         (void *)out, (void *)status})
             if (p) (void)hipFree(p);
         qa = ya = nullptr;
         acc = beta = qsum = nullptr;
         ysum = nullptr;
+        sync = nullptr;
         slots = out = nullptr;
         status = nullptr;
         cap = 0;
@@ -823,7 +931,8 @@ struct Context {
         slot_stride = round_up(ALTO_FRAME_OFFSET + e->request_len(), 256);
         bool ok = HIP_OK(hipMalloc(&qa, cap * 4 * e->rows)) && HIP_OK(hipMalloc(&ya, cap * 4 * e->kp)) &&
                   HIP_OK(hipMalloc(&acc, cap * e->cols * 4)) && HIP_OK(hipMalloc(&beta, cap * e->cols * 4)) &&
-                  HIP_OK(hipMalloc(&qsum, cap * 4)) && HIP_OK(hipMalloc(&ysum, cap * 8));
+                  HIP_OK(hipMalloc(&qsum, cap * 4)) && HIP_OK(hipMalloc(&ysum, cap * 8)) &&
+                  HIP_OK(hipMalloc(&sync, sync_words(*e) * 4));
         if (ok && with_host)
             ok = HIP_OK(hipMalloc(&slots, cap * slot_stride)) && HIP_OK(hipMalloc(&out, cap * e->response_len())) &&
                  HIP_OK(hipMalloc(&status, cap * 4));
@@ -831,7 +940,7 @@ struct Context {
         return ok;
     }
 
-    Workspace ws() const { return Workspace{qa, ya, acc, beta, qsum, ysum}; }
+    Workspace ws() const { return Workspace{qa, ya, acc, beta, qsum, ysum, sync}; }
     int enqueue(const Workspace &w, FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx,
                 int *status, hipStream_t st, bool timed, bool record_start = true);
     int answer_device(FrameSet fs, size_t B, uint8_t *out, size_t out_stride, const int *oidx, int *status,
@@ -848,7 +957,7 @@ Engine::~Engine() {
 
 size_t workspace_bytes(const Engine &e, size_t Bp) {
     return round_up(Bp * 4 * e.rows, 256) + round_up(Bp * 4 * e.kp, 256) + 2 * round_up(Bp * e.cols * 4, 256) +
-           round_up(Bp * 4, 256) + round_up(Bp * 8, 256);
+           round_up(Bp * 4, 256) + round_up(Bp * 8, 256) + round_up(sync_words(e) * 4, 256);
 }
 
 Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base) {
@@ -865,6 +974,7 @@ Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base) {
     w.beta = (uint32_t *)take(Bp * e.cols * 4);
     w.qsum = (uint32_t *)take(Bp * 4);
     w.ysum = (unsigned long long *)take(Bp * 8);
+    w.sync = (uint32_t *)take(sync_words(e) * 4);
     return w;
 }
 
@@ -872,7 +982,7 @@ Workspace workspace_at(const Engine &e, size_t Bp, uint8_t *base) {
 // the two contractions (or both in one launch), tail.  ev: the stage events to record at the
 // launch boundaries (null: none); *fused is set when both contractions ran in one launch.
 int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, size_t out_stride,
-                 const int *oidx, int *st_dev, hipStream_t st, hipEvent_t *ev, bool record_start, bool *fused,
+                 const int *oidx, int *st_dev, hipStream_t st, hipEvent_t *ev, bool record_start, int *plan,
                  const int *dims = nullptr) {
     int8_t *const qa = w.qa, *const ya = w.ya;
     uint32_t *const acc = w.acc, *const beta = w.beta, *const qsum = w.qsum;
@@ -919,23 +1029,14 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     pa.px = (unsigned)((E.KCp * 4 + 255) / 256);
     pa.py = (unsigned)Bp;
     pa.nfill = (unsigned)std::min<size_t>(1024, (pa.acc4 + 1023) / 1024);
+    pa.sync = w.sync;
+    pa.nsync = (unsigned)sync_words(E);
     pa.dims = dims;
     const unsigned prep_blocks = pa.sx * pa.sy + pa.px * pa.py + pa.nfill;
     hipLaunchKernelGGL(prep_kernel, dim3(prep_blocks), dim3(256), 0, st, pa);
     if (ev) (void)hipEventRecord(ev[2], st);
     const int fz = E.fused_for((int)MT);
-    if (fz >= 0) {
-        // one launch for both contractions; the stage times show the pair under "scan"
-        // (packing preparation and packing read as zero)
-        kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
-        if (ev) (void)hipEventRecord(ev[3], st);
-    } else {
-        kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
-        if (ev) (void)hipEventRecord(ev[3], st);
-        kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
-        if (ev) (void)hipEventRecord(ev[5], st);
-    }
-    *fused = fz >= 0;
+    *plan = fz;
     TailArgs ta;
     ta.fs = fs;
     ta.status = st_dev;
@@ -959,9 +1060,33 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     ta.q1_bits = E.q1_bits;
     ta.q1 = (uint32_t)E.q1;
     ta.dims = dims;
-    const size_t lds = tail_lds_bytes(E.n, E.part_words);
-    hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(256), lds, st, ta);
-    if (ev) (void)hipEventRecord(ev[6], st);
+    if (fz >= kNumFused) {
+        // scan, packing and tail in one launch; the stage times show it under "scan"
+        StepArgs sa{w.sync, (int)E.outputs, (int)E.n, 0, 0, (int)B, kStepSpin};
+        if (kStep[fz - kNumFused].launch(gs, qa, E.db, gp, ya, E.u, ta, sa, st) != 0) {
+            set_err("step kernel: LDS of the tail tasks exceeds the workgroup limit");
+            return -1;
+        }
+        if (ev) {
+            (void)hipEventRecord(ev[3], st);
+            (void)hipEventRecord(ev[6], st);
+        }
+    } else {
+        if (fz >= 0) {
+            // one launch for both contractions; the stage times show the pair under "scan"
+            // (packing preparation and packing read as zero)
+            kFused[fz].launch(gs, qa, E.db, gp, ya, E.u, st);
+            if (ev) (void)hipEventRecord(ev[3], st);
+        } else {
+            kVariants[E.variant_for(0, (int)MT)].scan(gs, qa, E.db, st);
+            if (ev) (void)hipEventRecord(ev[3], st);
+            kVariants[E.variant_for(1, (int)MT)].pack(gp, ya, E.u, st);
+            if (ev) (void)hipEventRecord(ev[5], st);
+        }
+        const size_t lds = tail_lds_bytes(E.n, E.part_words);
+        hipLaunchKernelGGL(tail_kernel, dim3((unsigned)E.outputs, (unsigned)B), dim3(kTailThreads), lds, st, ta);
+        if (ev) (void)hipEventRecord(ev[6], st);
+    }
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
         set_err("kernel launch failed: %s", hipGetErrorString(le));
@@ -972,10 +1097,10 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
 
 int Context::enqueue(const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, size_t out_stride, const int *oidx,
                      int *st_dev, hipStream_t st, bool timed, bool record_start) {
-    bool fused = false;
+    int plan = -1;
     const int rc = launch_batch(*e, w, fs, B, outp, out_stride, oidx, st_dev, st, timed ? ev : nullptr, record_start,
-                                &fused);
-    if (timed) ring_mask[ring_head] = fused ? kMaskFused : kMaskSplit;
+                                &plan);
+    if (timed) ring_mask[ring_head] = plan >= 0 ? kMaskFused : kMaskSplit;
     timed_valid = timed;
     return rc;
 }
@@ -983,7 +1108,8 @@ int Context::enqueue(const Workspace &w, FrameSet fs, size_t B, uint8_t *outp, s
 // Inserts the stage events of a batch into its captured launch chain as event-record nodes,
 // at the positions where an eager batch records them: the chain is validate, prep, scan,
 // packing, tail (fused: validate, prep, scan + packing, tail).
-bool Context::add_stage_events(hipGraph_t graph, bool fused, int set) {
+bool Context::add_stage_events(hipGraph_t graph, int plan, int set) {
+    const bool fused = plan >= 0, step = plan >= kNumFused;
     hipEvent_t *evs = ring[set];
     std::vector<hipGraphNode_t> chain;
     hipGraphNode_t node = nullptr;
@@ -997,14 +1123,15 @@ bool Context::add_stage_events(hipGraph_t graph, bool fused, int set) {
         if (nd != 1 || chain.size() > 16) return false;
         if (!HIP_OK(hipGraphNodeGetDependentNodes(node, &node, &nd))) return false;
     }
-    const int want = fused ? 4 : 5;
+    const int want = step ? 3 : fused ? 4 : 5;
     if ((int)chain.size() != want) {
         set_err("graph timing: %d nodes in the captured batch, expected %d", (int)chain.size(), want);
         return false;
     }
     // event k is recorded after chain node after[k] (-1: before the first, -2: not recorded)
-    static const int after_split[7] = {-1, 0, 1, 2, -2, 3, 4}, after_fused[7] = {-1, 0, 1, 2, -2, -2, 3};
-    const int *after = fused ? after_fused : after_split;
+    static const int after_split[7] = {-1, 0, 1, 2, -2, 3, 4}, after_fused[7] = {-1, 0, 1, 2, -2, -2, 3},
+                     after_step[7] = {-1, 0, 1, 2, -2, -2, 2};
+    const int *after = step ? after_step : fused ? after_fused : after_split;
     ring_mask[set] = fused ? kMaskFused : kMaskSplit;
     for (size_t i = 0; i + 1 < chain.size(); i++)
         if (!HIP_OK(hipGraphRemoveDependencies(graph, &chain[i], &chain[i + 1], 1))) return false;
@@ -1082,13 +1209,13 @@ int Context::answer_device(FrameSet fs, size_t B, uint8_t *outp, size_t out_stri
     int rc;
     if (bucketed) {
         hipLaunchKernelGGL(batch_dims_kernel, dim3(1), dim3(64), 0, st, dims, (int)Bg, MTg);
-        bool fused = false;
-        rc = launch_batch(*e, ws(), fs, Bg, outp, out_stride, oidx, st_dev, st, nullptr, false, &fused, dims);
+        int plan = -1;
+        rc = launch_batch(*e, ws(), fs, Bg, outp, out_stride, oidx, st_dev, st, nullptr, false, &plan, dims);
     } else {
         rc = enqueue(ws(), fs, B, outp, out_stride, oidx, st_dev, st, false);
     }
     const bool ended = HIP_OK(hipStreamEndCapture(st, &graph));
-    if (rc || !ended || (tg && !add_stage_events(graph, fz >= 0, rs))) {
+    if (rc || !ended || (tg && !add_stage_events(graph, fz, rs))) {
         if (graph) (void)hipGraphDestroy(graph);
         return -1;
     }
@@ -1515,6 +1642,9 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
     for (const FusedPair &f : kFused) {
         f.prepare();
     }
+    for (const StepPair &f : kStep) {
+        f.prepare();
+    }
     // resident state
     const size_t dbb = rows * cols, ubb = 4 * e->kp * cols;
     ok = ok && HIP_OK(hipMalloc(&e->db, dbb)) && HIP_OK(hipMalloc(&e->u, ubb)) &&
@@ -1667,9 +1797,9 @@ int alto_answer_device_noalloc(void *p, const uint8_t *slots, size_t slot_stride
     }
     if (!HIP_OK(hipSetDevice(e->device))) return -1;
     hipStream_t st = stream ? (hipStream_t)stream : e->ctx0->stream;
-    bool fused = false;
+    int plan = -1;
     return launch_batch(*e, workspace_at(*e, Bp, (uint8_t *)base), FrameSet{slots, slot_stride, nullptr}, B, out,
-                        out_stride, nullptr, status, st, nullptr, false, &fused);
+                        out_stride, nullptr, status, st, nullptr, false, &plan);
 }
 
 void *alto_context_new(void *p) {
@@ -1826,7 +1956,7 @@ int alto_set_graph_timing(void *p, int on) {
 int alto_set_fusion(void *p, int on) {
     auto *e = (Engine *)p;
     if (!e) return -1;
-    if (on < -1 || on > 1) return -1;
+    if (on < -1 || on > 2) return -1;
     const int prev = e->fuse;
     e->fuse = on;
     return prev;
@@ -1835,7 +1965,8 @@ int alto_set_fusion(void *p, int on) {
 int alto_fused(const void *p, size_t B) {
     auto *e = (const Engine *)p;
     if (!e || !B) return -1;
-    return e->fused_for((int)(round_up(B, 4) / 4)) >= 0 ? 1 : 0;
+    const int fz = e->fused_for((int)(round_up(B, 4) / 4));
+    return fz >= kNumFused ? 2 : fz >= 0 ? 1 : 0;
 }
 
 int alto_variant_count(void) { return kNumVariants; }

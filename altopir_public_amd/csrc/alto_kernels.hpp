@@ -379,7 +379,8 @@ __global__ __launch_bounds__(256) void limb_prep_kernel(FrameSet fs, size_t word
 // One launch for the per-batch preparation after validation: blocks [0, sx * sy) prepare
 // the query limbs (limb_prep_block<true>, grid sx x sy), the next px * py the key limbs
 // (limb_prep_block<false>), the last nfill clear the scan accumulators (acc4 16-byte
-// words).  The three parts are independent, so they run side by side.
+// words) and the first of them the step kernel's counters.  The three parts are
+// independent, so they run side by side.
 struct PrepArgs {
     FrameSet fs;
     const int *status;
@@ -391,6 +392,8 @@ struct PrepArgs {
     uint4 *acc;
     size_t acc4;
     unsigned sx, sy, px, py, nfill;
+    uint32_t *sync;   // the step kernel's counters, cleared by the first fill block
+    unsigned nsync;
     const int *dims;  // {B, MT} in device memory (launch sized for the batch's bucket), or null
 };
 __global__ __launch_bounds__(256) void prep_kernel(PrepArgs p) {
@@ -406,6 +409,8 @@ __global__ __launch_bounds__(256) void prep_kernel(PrepArgs p) {
         return;
     }
     b -= p.px * p.py;
+    if (b == 0)
+        for (unsigned i = threadIdx.x; i < p.nsync; i += blockDim.x) p.sync[i] = 0;
     const uint4 z = make_uint4(0, 0, 0, 0);
     for (size_t i = (size_t)b * blockDim.x + threadIdx.x; i < p.acc4; i += (size_t)p.nfill * blockDim.x) p.acc[i] = z;
 }
@@ -1404,32 +1409,37 @@ __device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *(const uint4 *
 // LDS: a[n] | twiddles[n] | part[part_words] (tail_lds_bytes).
 __host__ __device__ constexpr size_t tail_lds_bytes(size_t n, size_t part_words) { return (2 * n + part_words) * 4; }
 
-__global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
-    extern __shared__ uint32_t lds[];
-    const int o = blockIdx.x, c = blockIdx.y;
-    if (p.dims && c >= p.dims[0]) return;
-    if (p.status[c] != 0) return;
+constexpr int kTailThreads = 256;  // threads of one tail task
+
+// The tail of (output o, query c) on kTailThreads threads (tid < kTailThreads) of a
+// workgroup, with its LDS at lds (tail_lds_bytes).  The barriers are the workgroup's: every
+// thread of the workgroup calls this the same number of times; a group of threads whose task
+// is inactive (a row past the batch, a rejected request, no task) only takes part in them.
+__device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool active, uint32_t *lds, int tid) {
+    constexpr int NTH = kTailThreads;
     const int n = p.n, logn = p.logn, groups = n >> 3;
     uint32_t *a = lds;
     uint32_t *tw = lds + n;
     uint32_t *part = lds + 2 * n;
     const size_t col0 = (size_t)o * n;
     const uint32_t *pre = (const uint32_t *)(p.prefix + (size_t)o * p.part_words * 4);
-    for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) part[w] = pre[w];
-    const uint32_t *zk = (const uint32_t *)(p.fs.slot(c) + 16) + p.kp;
+    const uint32_t *zk = active ? (const uint32_t *)(p.fs.slot(c) + 16) + p.kp : nullptr;
     const uint32_t *th = p.that + (size_t)o * p.kp;
     const uint32_t *bc = p.beta + (size_t)c * p.C + col0;
     const uint32_t *accc = p.acc + (size_t)c * p.C + col0;
-    const uint32_t qs = p.qsum[c] << 7;
-    // n is a power of two of at least 16
-    for (int w = threadIdx.x; w < (n >> 2); w += blockDim.x) ((uint4 *)tw)[w] = ld4(p.wtab + 4 * w);
+    const uint32_t qs = active ? p.qsum[c] << 7 : 0;
+    if (active) {
+        for (int w = tid; w < p.part_words; w += NTH) part[w] = pre[w];
+        // n is a power of two of at least 16
+        for (int w = tid; w < (n >> 2); w += NTH) ((uint4 *)tw)[w] = ld4(p.wtab + 4 * w);
+    }
     // phase 0 takes the r0 = logn - 3 (nph - 1) first stages (1..3) on contiguous groups, so
     // that every later phase has exactly three stages and its groups (stride 2^ls, 8
     // elements) stay inside the n points
     const int nph = (logn + 2) / 3, r0 = logn - 3 * (nph - 1);
     for (int ph = 0; ph < nph; ph++) {
         const int ls = ph == 0 ? 0 : r0 + 3 * (ph - 1), sp = 1 << ls, nst = ph == 0 ? r0 : 3;
-        for (int g = threadIdx.x; g < groups; g += blockDim.x) {
+        for (int g = tid; active && g < groups; g += NTH) {
             const int base = ((g >> ls) << (ls + 3)) | (g & (sp - 1));
             uint32_t v[8];
             if (ph == 0) {
@@ -1504,12 +1514,161 @@ __global__ __launch_bounds__(256) void tail_kernel(TailArgs p) {
         }
         __syncthreads();
     }
+    if (!active) return;
     uint8_t *dst = p.out + (size_t)(p.oidx ? p.oidx[c] : c) * p.out_stride + (size_t)o * p.part_len;
     if ((((uintptr_t)dst) & 3) == 0 && (p.part_len & 3) == 0) {
         uint32_t *d32 = (uint32_t *)dst;
-        for (int w = threadIdx.x; w < p.part_words; w += blockDim.x) d32[w] = part[w];
+        for (int w = tid; w < p.part_words; w += NTH) d32[w] = part[w];
     } else {
-        for (int i = threadIdx.x; i < p.part_len; i += blockDim.x) dst[i] = (uint8_t)(part[i >> 2] >> (8 * (i & 3)));
+        for (int i = tid; i < p.part_len; i += NTH) dst[i] = (uint8_t)(part[i >> 2] >> (8 * (i & 3)));
+    }
+}
+
+// Whether the tail of query c runs: inside the batch and accepted by validation.
+__device__ __forceinline__ bool tail_wanted(const TailArgs &p, int c) {
+    return !(p.dims && c >= p.dims[0]) && p.status[c] == 0;
+}
+
+__global__ __launch_bounds__(kTailThreads) void tail_kernel(TailArgs p) {
+    extern __shared__ uint32_t lds[];
+    const int o = blockIdx.x, c = blockIdx.y;
+    if (!tail_wanted(p, c)) return;  // the whole workgroup: no barrier is left waiting
+    tail_task(p, o, c, true, lds, threadIdx.x);
+}
+
+// ------------------------------------------------------- scan, packing and tail in one launch --
+
+// A batch as one launch: the workgroups of both contractions (as scan_pack_kernel) and,
+// behind them, the tails.  Output part o (columns [o n, (o + 1) n)) is complete when every
+// workgroup whose column range meets it has stored its results: each workgroup, once its
+// tile is stored, releases its stores and adds one arrival to each part its columns meet.
+// The workgroup that brings a part's arrivals to their total (known in closed form from the
+// two grids) publishes the part's B tail tasks, one per query.  Tail tasks are then claimed
+// with an atomic counter per part, kTailThreads threads per task (two tasks at a time in a
+// workgroup of 512 threads), by
+//   * the publisher, which claims from its own part first and only leaves once no published
+//     part has a task left (so every published task is run even if nobody else helps), and
+//   * every other workgroup once its tile is done, which takes published tasks before it
+//     leaves and, once every workgroup of the grid has started, waits for parts still to be
+//     published (polling with s_sleep, at most `spin` polls) until every part's tasks are
+//     claimed.
+// No workgroup waits while any workgroup of the grid has yet to start, so the waits never
+// hold a CU that an unstarted tile needs, and the tiles themselves never wait: every wait
+// ends, and the bounded poll count is only a fail-safe (a helper that gives up changes no
+// result, the publishers drain their parts).  Part completion is known per part over all
+// row blocks.
+// sync (3 parts + 4 words, zero before the launch): [0, O) arrivals, [O, 2O) claimed tasks,
+// [2O, 3O) published flags, [3O] workgroups started, [3O + 1] parts whose tasks are all
+// claimed, [3O + 2] set when a helper gave up waiting.
+struct StepArgs {
+    uint32_t *sync;
+    int parts, part_cols;       // O, n
+    int scan_cols, pack_cols;   // columns of one x tile of the scan / packing grid
+    int B;                      // tail tasks per part (batch size; dims[0] when given)
+    int spin;                   // polls a waiting helper makes at most
+};
+
+// LDS of one tail task inside the step kernel, in words, rounded to 16 bytes; the kernel
+// needs step_lds_bytes(H, ...) of dynamic LDS at least (and the tiles' own).
+__host__ __device__ constexpr int tail_step_words(int n, int part_words) { return (2 * n + part_words + 3) & ~3; }
+__host__ __device__ constexpr size_t step_lds_bytes(int H, int n, int part_words) {
+    return ((size_t)H * tail_step_words(n, part_words) + 4) * 4;
+}
+
+__device__ __forceinline__ int tiles_meeting_part(int p, int n, int cw, int gx) {
+    const int x0 = (p * n) / cw, x1 = min((p * n + n - 1) / cw, gx - 1);
+    return x1 >= x0 ? x1 - x0 + 1 : 0;
+}
+
+template <class Scan, class Pack>
+__global__ __launch_bounds__(Scan::kThreads, 1) void step_kernel(const i32x4 *__restrict__ qa,
+                                                                 const i32x4 *__restrict__ db, GemmArgs gs,
+                                                                 const i32x4 *__restrict__ ya,
+                                                                 const i32x4 *__restrict__ u, GemmArgs gp,
+                                                                 int scan_blocks, TailArgs ta, StepArgs sa) {
+    static_assert(Scan::kThreads == Pack::kThreads, "one workgroup size");
+    static_assert(Scan::kThreads % kTailThreads == 0, "whole tail tasks per workgroup");
+    constexpr int H = Scan::kThreads / kTailThreads;  // tail tasks a workgroup runs at once
+    // dynamic LDS: the tiles' rings while they run, then H tail regions of tail_step_words
+    // words each and the claimed group {part, first query} (-1: none) behind them
+    extern __shared__ uint32_t step_lds[];
+    const int tw_words = tail_step_words(ta.n, ta.part_words);
+    int *task = (int *)(step_lds + H * tw_words);
+    const int O = sa.parts, n = sa.part_cols;
+    uint32_t *arrive = sa.sync, *claimed = sa.sync + O, *ready = sa.sync + 2 * O;
+    uint32_t *started = sa.sync + 3 * O, *parts_done = started + 1, *gave_up = started + 2;
+    const int L = blockIdx.x;
+    const bool scan = L < scan_blocks;
+    if (threadIdx.x == 0) atomicAdd(started, 1u);
+    if (scan) Scan::tile(qa, db, gs, L);
+    else Pack::tile(ya, u, gp, L - scan_blocks);
+    const int B = ta.dims ? ta.dims[0] : sa.B;
+    // every wave's stores are done (the barrier), then one release at agent scope before
+    // the arrivals
+    __syncthreads();
+    int own = -1;
+    if (threadIdx.x == 0) {
+        const GemmArgs &g = scan ? gs : gp;
+        const TileMap tm = map_tile_at(scan ? L : L - scan_blocks, g.gx, g.gy, g.gz);
+        if (tm.valid) {
+            const int cw = scan ? sa.scan_cols : sa.pack_cols;
+            const int c0 = tm.x * cw, c1 = min((tm.x + 1) * cw, gs.C) - 1;
+            __threadfence();
+            for (int p = c0 / n; p <= c1 / n; p++) {
+                const uint32_t total = (uint32_t)(gs.gy * gs.gz * tiles_meeting_part(p, n, sa.scan_cols, gs.gx) +
+                                                  gp.gy * gp.gz * tiles_meeting_part(p, n, sa.pack_cols, gp.gx));
+                if (atomicAdd(&arrive[p], 1u) + 1 == total) {
+                    __threadfence();  // acquire the other workgroups' results, release them on
+                    __hip_atomic_store(&ready[p], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    if (own < 0) own = p;
+                }
+            }
+        }
+    }
+    // claim groups of H tasks from published parts until none is left (and, once the whole
+    // grid has started, until every part is claimed)
+    const int start = own >= 0 ? own : L % O;
+    int scanned = 0, polls = 0;
+    while (true) {
+        if (threadIdx.x == 0) {
+            int tp = -1, tc = -1;
+            while (tp < 0) {
+                for (; scanned < O; scanned++) {
+                    const int p = (start + scanned) % O;
+                    if (!__hip_atomic_load(&ready[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) continue;
+                    if (__hip_atomic_load(&claimed[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)B)
+                        continue;
+                    const uint32_t t = atomicAdd(&claimed[p], (uint32_t)H);
+                    if (t >= (uint32_t)B) continue;
+                    if (t + H >= (uint32_t)B) atomicAdd(parts_done, 1u);
+                    tp = p;
+                    tc = (int)t;
+                    break;
+                }
+                if (tp >= 0) break;
+                // nothing published is left: wait only once every workgroup has started and
+                // while some part still has tasks to publish or claim
+                if (__hip_atomic_load(parts_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)O ||
+                    __hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x)
+                    break;
+                if (++polls > sa.spin) {
+                    atomicOr(gave_up, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+                scanned = 0;
+            }
+            if (tp >= 0) __threadfence();  // acquire: the part's results before its tails read them
+            task[0] = tp;
+            task[1] = tc;
+        }
+        __syncthreads();
+        const int tp = task[0], tc = task[1];
+        if (tp < 0) break;
+        const int h = threadIdx.x / kTailThreads, c = tc + h;
+        const bool act = c < B && tail_wanted(ta, c);
+        tail_task(ta, tp, act ? c : 0, act, step_lds + (size_t)h * tw_words, threadIdx.x % kTailThreads);
+        __syncthreads();  // the task slot and the LDS are reused by the next group
     }
 }
 

@@ -297,13 +297,19 @@ class Engine:
     def set_fusion(self, on) -> None:
         """One launch for scan and packing where the chosen shape pair allows: True / 1 for
         every such pair, False / 0 never, -1 the built-in choice (batches of 1-4 queries on
-        databases of at most 4 KP records)."""
-        if self._lib.alto_set_fusion(self.ptr, int(on)) < -1:
+        databases of at most 4 KP records); 2: scan, packing and tail in one launch (the
+        step kernel) for every batch size whose shapes have a step pair, else as 1."""
+        if int(on) not in (-1, 0, 1, 2):
             raise ValueError(on)
+        self._lib.alto_set_fusion(self.ptr, int(on))
 
     def fused(self, b: int) -> bool:
         """Whether a batch of b queries runs its two contractions in one launch."""
-        return self._lib.alto_fused(self.ptr, b) == 1
+        return self._lib.alto_fused(self.ptr, b) >= 1
+
+    def step(self, b: int) -> bool:
+        """Whether a batch of b queries runs scan, packing and tail in one launch."""
+        return self._lib.alto_fused(self.ptr, b) == 2
 
     def last_timings(self) -> list:
         ms = (C.c_double * 6)()

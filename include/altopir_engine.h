@@ -214,10 +214,17 @@ int alto_set_graph_timing(void *e, int on);
  * on: 1 fuse every such pair; 0 never; -1 (the default) only batches of 1-4 queries on
  * databases of at most 4 * tg * n records, where one launch of the two streaming
  * contractions measured 4% faster on MI355X (elsewhere it was level or slower: each
- * contraction alone fills the chip).  alto_set_fusion returns the
- * previous setting (-1, 0 or 1), -1 also on a null engine or a bad value;
- * alto_fused returns 1 if a batch of B queries runs fused with the current settings, 0 if
- * not, -1 on bad arguments. */
+ * contraction alone fills the chip).
+ * on = 2: the whole batch after validation and limb preparation as one persistent launch
+ * (the step kernel): scan and packing tiles, and, as each output part's tiles have all
+ * stored their results (counted per part in the workspace), that part's tails, one per
+ * query, claimed by the workgroups whose tiles are done; the stage timings report it under
+ * "scan" and zero for packing and tail.  Every batch size has a step pair in the built-in
+ * policy (for 9-16 queries the packing then runs on 8 x 16-tile workgroups, the scan's
+ * workgroup size); batch sizes whose tuned shapes have none run as with on = 1.
+ * alto_set_fusion returns the previous setting (-1 to 2), -1 also on a null engine or a
+ * bad value; alto_fused returns 2 if a batch of B queries runs as one step launch, 1 if its
+ * two contractions run in one launch, 0 if not, -1 on bad arguments. */
 int alto_set_fusion(void *e, int on);
 int alto_fused(const void *e, size_t B);
 
