@@ -480,8 +480,8 @@ const StepPair kStep[] = {
 };
 constexpr int kNumStep = (int)(sizeof(kStep) / sizeof(kStep[0]));
 // Polls a helper workgroup of the step kernel makes at most while it waits for parts to be
-// published (about 0.3 ms each 1,000; the waits end long before in every correct run).
-constexpr int kStepSpin = 1 << 20;
+// published (about 1 ms each 1,000; the waits end long before in every correct run).
+constexpr int kStepSpin = 1 << 18;
 
 // Built-in shape policy (used until the tuner records a choice for a batch-size bucket),
 // from sweeps over batch sizes 1-2048 on 1-64 GiB databases: streaming kernels up to 8

@@ -1415,6 +1415,8 @@ constexpr int kTailThreads = 256;  // threads of one tail task
 // workgroup, with its LDS at lds (tail_lds_bytes).  The barriers are the workgroup's: every
 // thread of the workgroup calls this the same number of times; a group of threads whose task
 // is inactive (a row past the batch, a rejected request, no task) only takes part in them.
+// KB: keys whose loads are issued together in the pointwise step.
+template <int KB = 4>
 __device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool active, uint32_t *lds, int tid) {
     constexpr int NTH = kTailThreads;
     const int n = p.n, logn = p.logn, groups = n >> 3;
@@ -1448,11 +1450,11 @@ __device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool 
                 const uint4 b0 = ld4(bc + base), b1 = ld4(bc + base + 4);
                 s8[0] = b0.x; s8[1] = b0.y; s8[2] = b0.z; s8[3] = b0.w;
                 s8[4] = b1.x; s8[5] = b1.y; s8[6] = b1.z; s8[7] = b1.w;
-                // keys four at a time: all their loads are issued before the first product
-                for (int k0 = 0; k0 < p.tg; k0 += 4) {
-                    uint4 z0[4], z1[4], t0[4], t1[4];
+                // keys KB at a time: all their loads are issued before the first product
+                for (int k0 = 0; k0 < p.tg; k0 += KB) {
+                    uint4 z0[KB], z1[KB], t0[KB], t1[KB];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
+                    for (int j = 0; j < KB; j++) {
                         const int k = min(k0 + j, p.tg - 1);
                         z0[j] = ld4(zk + k * n + base);
                         z1[j] = ld4(zk + k * n + base + 4);
@@ -1460,7 +1462,7 @@ __device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool 
                         t1[j] = ld4(th + k * n + base + 4);
                     }
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
+                    for (int j = 0; j < KB; j++) {
                         if (k0 + j >= p.tg) break;
                         s8[0] += mulmod(z0[j].x, t0[j].x); s8[1] += mulmod(z0[j].y, t0[j].y);
                         s8[2] += mulmod(z0[j].z, t0[j].z); s8[3] += mulmod(z0[j].w, t0[j].w);
@@ -1550,8 +1552,8 @@ __global__ __launch_bounds__(kTailThreads) void tail_kernel(TailArgs p) {
 //     part has a task left (so every published task is run even if nobody else helps), and
 //   * every other workgroup once its tile is done, which takes published tasks before it
 //     leaves and, once every workgroup of the grid has started, waits for parts still to be
-//     published (polling with s_sleep, at most `spin` polls) until every part's tasks are
-//     claimed.
+//     published (polling relaxed flags with s_sleep, at most `spin` polls) until every
+//     part's tasks are claimed; only when a part has more tasks than one claim takes.
 // No workgroup waits while any workgroup of the grid has yet to start, so the waits never
 // hold a CU that an unstarted tile needs, and the tiles themselves never wait: every wait
 // ends, and the bounded poll count is only a fail-safe (a helper that gives up changes no
@@ -1635,7 +1637,9 @@ __global__ __launch_bounds__(Scan::kThreads, 1) void step_kernel(const i32x4 *__
             while (tp < 0) {
                 for (; scanned < O; scanned++) {
                     const int p = (start + scanned) % O;
-                    if (!__hip_atomic_load(&ready[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) continue;
+                    // relaxed: an acquiring load invalidates the L2 lines the other workgroups
+                    // of this XCD stream through; the acquire follows a successful claim
+                    if (!__hip_atomic_load(&ready[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
                     if (__hip_atomic_load(&claimed[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)B)
                         continue;
                     const uint32_t t = atomicAdd(&claimed[p], (uint32_t)H);
@@ -1646,19 +1650,23 @@ __global__ __launch_bounds__(Scan::kThreads, 1) void step_kernel(const i32x4 *__
                     break;
                 }
                 if (tp >= 0) break;
-                // nothing published is left: wait only once every workgroup has started and
-                // while some part still has tasks to publish or claim
-                if (__hip_atomic_load(parts_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)O ||
+                // nothing published is left: wait only when a part holds more than one claim of
+                // tasks (else its publisher runs them all), once every workgroup has started,
+                // and while some part still has tasks to publish or claim
+                if (B <= H ||
+                    __hip_atomic_load(parts_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)O ||
                     __hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x)
                     break;
                 if (++polls > sa.spin) {
                     atomicOr(gave_up, 1u);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(8);
+                __builtin_amdgcn_s_sleep(32);
                 scanned = 0;
             }
-            if (tp >= 0) __threadfence();  // acquire: the part's results before its tails read them
+            // acquire (after the relaxed read of the published flag): the part's results
+            // before its tails read them
+            if (tp >= 0) __threadfence();
             task[0] = tp;
             task[1] = tc;
         }
