@@ -1,0 +1,100 @@
+"""The server's offline setup for the scheme of client.py: hint, packing precomputation, engine.
+
+From a database and the public values of client.py the server computes (src/service.rs:
+129-170, Prepared::new with perform_offline_precomputation_simplepir, src/server.rs:947-1021):
+  * the hint: for every column, the sum over the record polynomials of their products with the
+    public query polynomials (answer_hint_ring; alto_hint0 on the device);
+  * per output part, the InspiRING precomputation from the hint and the packing keys' masks:
+    the aggregates bold_t, bold_t_bar (one gadget-decomposed polynomial per generator power),
+    bold_t_hat and the packed ciphertext's first row a_hat (src/packing.rs:2260-2359;
+    alto_inspiring_offline on the device);
+  * the packing matrix U from bold_t / bold_t_bar (sw_pack_build; alto_pack_build on the
+    device), the tail keys (bold_t_hat), the row-0 prefixes (a_hat switched to q2 = 2^18,
+    src/modulus_switch.rs:136-147) and the inverse-NTT table.
+server_inputs() assembles the engine's inputs from these (workload.EngineInputs).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib, client as cl, workload
+
+Q = workload.Q
+
+
+def automorph_tables(n: int) -> np.ndarray:
+    """Every automorphism's slot table, table (t - 1) / 2 for odd t < 2n, n x n u32
+    (src/packing.rs:1082-1110, the layout sw_pack_build reads)."""
+    return np.stack([cl.auto_slots(t, n) for t in range(1, 2 * n, 2)]).astype(np.uint32).reshape(-1)
+
+
+def write_fields(buf: np.ndarray, bit0: int, bits: int, vals: np.ndarray) -> None:
+    """ORs LSB-first fields of `bits` bits from bit offset bit0 into buf (src/bits.rs:11-41)."""
+    vals = np.asarray(vals, dtype=np.uint64)
+    b = ((vals[:, None] >> np.arange(bits, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8).reshape(-1)
+    allbits = np.unpackbits(buf, bitorder="little")
+    allbits[bit0:bit0 + len(b)] |= b
+    buf[:] = np.packbits(allbits, bitorder="little")[:len(buf)]
+
+
+def prefix_from_a_hat(a_hat: np.ndarray, n: int) -> np.ndarray:
+    """Row 0 of a response part: a_hat rescaled to q2 = 2^18, 18-bit fields from bit 0
+    (switch_prefix_row0, src/modulus_switch.rs:136-147); row 1 left zero."""
+    buf = np.zeros(workload.part_len_for(n), np.uint8)
+    write_fields(buf, 0, 18, cl.rescale(a_hat, Q, cl.Q2))
+    return buf
+
+
+def device_hint(params: cl.Params, db_col: np.ndarray, ld: int | None = None) -> np.ndarray:
+    """The hint (n x cols) computed on the device from a column-major database (alto_hint0)."""
+    ld = params.rows if ld is None else ld
+    a = np.ascontiguousarray(cl.hint_query_polys(params), dtype=np.uint64)
+    out = np.zeros((params.n, params.cols), np.uint64)
+    lib = _lib.load()
+    rc = lib.alto_hint0(db_col.ctypes.data, 0, ld, params.rows, params.cols, params.n, a.ctypes.data, a.shape[0],
+                        out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"alto_hint0 failed ({_lib.last_error()})")
+    return out
+
+
+def device_offline(params: cl.Params, hint: np.ndarray) -> list[tuple]:
+    """The packing precomputation of every output part on the device
+    (alto_inspiring_offline): a list of (bold_t, bold_t_bar, bold_t_hat, a_hat) per part."""
+    n, tg, O = params.n, params.tg, params.outputs
+    w, v = cl.masks(params)
+    hint = np.ascontiguousarray(hint, dtype=np.uint64)
+    m = n // 2 - 1
+    bt = np.zeros((O, m * tg * n), np.uint64)
+    bb = np.zeros((O, m * tg * n), np.uint64)
+    bh = np.zeros((O, tg * n), np.uint64)
+    ah = np.zeros((O, n), np.uint64)
+    lib = _lib.load()
+    rc = lib.alto_inspiring_offline(bt.ctypes.data, bb.ctypes.data, bh.ctypes.data, ah.ctypes.data,
+                                    hint.ctypes.data, params.cols, O, np.ascontiguousarray(w).ctypes.data,
+                                    np.ascontiguousarray(v).ctypes.data, n, tg)
+    if rc != 0:
+        raise RuntimeError(f"alto_inspiring_offline failed ({_lib.last_error()})")
+    return [(bt[o], bb[o], bh[o], ah[o]) for o in range(O)]
+
+
+def server_inputs(params: cl.Params, db_col: np.ndarray, parts: list[tuple], ub: np.ndarray | None = None,
+                  device: bool = True, threads: int = 16) -> workload.EngineInputs:
+    """The engine's inputs from the per-part precomputation: U built from bold_t / bold_t_bar
+    (on the device, or with sw_pack_build on host threads), that = bold_t_hat, the row-0
+    prefixes from a_hat, the inverse-NTT table of the client's NTT.  ub: a packing matrix
+    built already (skips the build)."""
+    from .engine import build_packing_matrix
+
+    n, tg = params.n, params.tg
+    if ub is None:
+        ub = np.zeros(4 * tg * n * params.cols, np.int8)
+        build_packing_matrix(ub, [p[0] for p in parts], [p[1] for p in parts], automorph_tables(n),
+                             cl.gen_pows(n).astype(np.uint32), n // 2 - 1, tg, n, Q, threads, device=device)
+    return workload.EngineInputs(rows=params.rows, cols=params.cols, n=n, tg=tg, ld=params.rows, db_col=db_col,
+                                 ub=ub, prefixes=[prefix_from_a_hat(p[3], n) for p in parts],
+                                 that=[np.ascontiguousarray(p[2], dtype=np.uint64) for p in parts],
+                                 inv_table=workload.inverse_table(n), part_len=workload.part_len_for(n),
+                                 row1_off=18 * n)

@@ -455,3 +455,135 @@ void orc_hint0(uint64_t *hint, size_t stride, const uint64_t *a, size_t npolys, 
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------- */
+/* Offline packing precomputation (InspiRING) of one output part.                          */
+
+/* Forward negacyclic NTT in place, Cooley-Tukey with the powers psi^bitrev(k) (k = m + i in
+ * stage m), the output in the slot order of orc_ntt_naive (slot j = evaluation at
+ * psi^(2 bitrev(j) + 1)): the transform whose inverse orc_intt computes with the table
+ * psi^-bitrev(k) / 2.  The reference takes it from spiral-rs (PolyMatrixRaw::ntt); it is
+ * pinned against orc_ntt_naive. */
+void orc_ntt(uint64_t *a, size_t n, uint64_t psi) {
+    size_t logn = 0;
+    while (((size_t)1 << logn) < n) logn++;
+    for (size_t i = 0; i < n; i++) a[i] %= ORC_Q;
+    size_t t = n;
+    for (size_t m = 1; m < n; m *= 2) {
+        t /= 2;
+        for (size_t i = 0; i < m; i++) {
+            const uint64_t s = orc_powmod(psi, bitrev(m + i, logn));
+            for (size_t j = 2 * i * t; j < 2 * i * t + t; j++) {
+                const uint64_t u = a[j], v = orc_mulmod(a[j + t], s);
+                a[j] = (u + v) % ORC_Q;
+                a[j + t] = (u + ORC_Q - v) % ORC_Q;
+            }
+        }
+    }
+}
+
+/* Slot permutation of the automorphism x -> x^t (t odd) in the NTT domain, the closed form
+ * of generate_automorph_tables_brute_force (src/packing.rs:1082-1110):
+ * tau_t(a)[j] = a[table_t[j]], table_t[j] = bitrev(((t (2 bitrev(j) + 1)) mod 2n - 1) / 2). */
+size_t orc_auto_slot(size_t t, size_t j, size_t n) {
+    size_t logn = 0;
+    while (((size_t)1 << logn) < n) logn++;
+    return bitrev(((t * (2 * bitrev(j, logn) + 1)) % (2 * n) - 1) / 2, logn);
+}
+
+/* Coefficients of a (NTT slots) with the table psi^-bitrev(k) / 2 (orc_intt). */
+static void intt_psi(uint64_t *a, size_t n, uint64_t psi) {
+    size_t logn = 0;
+    while (((size_t)1 << logn) < n) logn++;
+    uint64_t *tab = malloc(n * sizeof *tab);
+    const uint64_t psi_inv = orc_powmod(psi, ORC_Q - 2), half = (ORC_Q + 1) / 2;
+    for (size_t k = 0; k < n; k++) tab[k] = orc_mulmod(orc_powmod(psi_inv, bitrev(k, logn)), half);
+    orc_intt(a, tab, n);
+    free(tab);
+}
+
+/* Gadget decomposition of a coefficient polynomial c (values < Q) into tg digit polynomials
+ * of base 2^ceil(32 / tg), least significant first (spiral-rs gadget_invert_transposed,
+ * called at src/packing.rs:2323/2335/2343 on the raw polynomial), then each digit polynomial
+ * to NTT slots: out[k * n + slot]. */
+static void gadget_ntt(uint64_t *out, const uint64_t *c, size_t n, size_t tg, uint64_t psi) {
+    const unsigned bits = (unsigned)((32 + tg - 1) / tg);
+    const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1);
+    for (size_t k = 0; k < tg; k++) {
+        for (size_t z = 0; z < n; z++) out[k * n + z] = (k * bits < 64) ? (c[z] >> (k * bits)) & mask : 0;
+        orc_ntt(out + k * n, n, psi);
+    }
+}
+
+/* The offline packing precomputation of output part `part` (columns col0 = part n .. + n):
+ *   src/packing.rs:910-942 (prep_pack_lwes): the a-polynomial of column j is
+ *     a~_j = NTT(negacyclic_perm(hint[. , j]))  (hint row z, coefficient z);
+ *   :2260-2291 (precompute_aggregates_ntt): E[v][slot] = NTT(row_v)[slot] with
+ *     row_v[j] = a~_j[v], and for i < n/2, g = 5^i mod 2n:
+ *     r_i[slot] = n^-1 E[table_g[slot]][slot],  rbar_i[slot] = n^-1 E[table_{2n-g}[slot]][slot];
+ *   :2301-2359 (full_packing_offline_mode), walking i = n/2 - 2 .. 0:
+ *     bold_t[i] = NTT(G^-1(INTT(r_{i+1}))),  r_i += sum_k tau_g(w_k) . bold_t[i][k],
+ *     bold_t_bar[i] = NTT(G^-1(INTT(rbar_{i+1}))),  rbar_i += sum_k tau_{2n-g}(w_k) . bold_t_bar[i][k],
+ *   then bold_t_hat = NTT(G^-1(INTT(rbar_0))), r_0 += sum_k v_k . bold_t_hat[k],
+ *   a_hat = INTT(r_0) (coefficients).
+ * w, v: tg x n mask slots (the reference draws them from W_SEED / V_SEED, src/scheme.rs:24-31).
+ * Outputs: bold_t, bold_t_bar ((n/2 - 1) tg n values, [(i tg + k) n + slot]), bold_t_hat
+ * (tg n), a_hat (n).  hint[z * stride + col0 + j].  Returns 0, -1 for n not a power of two
+ * of at least 16. */
+int orc_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
+                          const uint64_t *hint, size_t stride, size_t col0, const uint64_t *w,
+                          const uint64_t *v, size_t n, size_t tg, uint64_t psi) {
+    if (n < 16 || (n & (n - 1))) return -1;
+    const size_t half = n / 2;
+    uint64_t *at = malloc(n * n * sizeof *at);   /* a~_j[slot] at [j n + slot] */
+    uint64_t *ev = malloc(n * n * sizeof *ev);   /* E[v][slot] */
+    uint64_t *r = malloc(half * n * sizeof *r), *rb = malloc(half * n * sizeof *rb);
+    uint64_t *c = malloc(n * sizeof *c);
+    for (size_t j = 0; j < n; j++) {
+        uint64_t *p = at + j * n;
+        for (size_t z = 0; z < n; z++) {  /* negacyclic_perm (src/util.rs:5-21) */
+            const uint64_t h = hint[(z == 0 ? 0 : n - z) * stride + col0 + j] % ORC_Q;
+            p[z] = z == 0 ? h : (ORC_Q - h) % ORC_Q;
+        }
+        orc_ntt(p, n, psi);
+    }
+    for (size_t vv = 0; vv < n; vv++) {
+        uint64_t *e = ev + vv * n;
+        for (size_t j = 0; j < n; j++) e[j] = at[j * n + vv];
+        orc_ntt(e, n, psi);
+    }
+    const uint64_t ninv = orc_powmod(n, ORC_Q - 2);
+    uint64_t g = 1;
+    for (size_t i = 0; i < half; i++) {
+        for (size_t s = 0; s < n; s++) {
+            r[i * n + s] = orc_mulmod(ninv, ev[orc_auto_slot(g, s, n) * n + s]);
+            rb[i * n + s] = orc_mulmod(ninv, ev[orc_auto_slot(2 * n - g, s, n) * n + s]);
+        }
+        g = g * 5 % (2 * n);
+    }
+    size_t *gp = malloc(half * sizeof *gp);  /* gen_pows, src/packing.rs:1695-1703 */
+    gp[0] = 1;
+    for (size_t i = 1; i < half; i++) gp[i] = gp[i - 1] * 5 % (2 * n);
+    for (size_t ii = half - 1; ii-- > 0;) {
+        for (int bar = 0; bar < 2; bar++) {
+            uint64_t *src = (bar ? rb : r) + (ii + 1) * n, *dst = (bar ? rb : r) + ii * n;
+            uint64_t *bt = (bar ? bold_t_bar : bold_t) + ii * tg * n;
+            const size_t t = bar ? 2 * n - gp[ii] : gp[ii];
+            memcpy(c, src, n * sizeof *c);
+            intt_psi(c, n, psi);
+            gadget_ntt(bt, c, n, tg, psi);
+            for (size_t k = 0; k < tg; k++)
+                for (size_t s = 0; s < n; s++)
+                    dst[s] = (dst[s] + orc_mulmod(w[k * n + orc_auto_slot(t, s, n)] % ORC_Q, bt[k * n + s])) % ORC_Q;
+        }
+    }
+    memcpy(c, rb, n * sizeof *c);
+    intt_psi(c, n, psi);
+    gadget_ntt(bold_t_hat, c, n, tg, psi);
+    for (size_t k = 0; k < tg; k++)
+        for (size_t s = 0; s < n; s++) r[s] = (r[s] + orc_mulmod(v[k * n + s] % ORC_Q, bold_t_hat[k * n + s])) % ORC_Q;
+    memcpy(a_hat, r, n * sizeof *a_hat);
+    intt_psi(a_hat, n, psi);
+    free(at); free(ev); free(r); free(rb); free(c); free(gp);
+    return 0;
+}

@@ -34,6 +34,11 @@
  *   orc_hint0         an independent evaluation (full linear convolution of each pair of
  *                     polynomials, then the fold x^n = -1), and the closed forms for the query
  *                     polynomials 1 and x (sums of record bytes, shifted with one negated term)
+ *   orc_ntt           orc_ntt_naive (slot evaluation), and orc_intt inverts it
+ *   orc_inspiring_offline  the aggregates against the reference's direct form (sums of
+ *                     monomial products, src/packing.rs:2179-2258, evaluated term by term);
+ *                     the whole precomputation by decryption: responses the engine computes
+ *                     from it decrypt, with the client's secret, to the database records
  */
 #ifndef ALTO_ORACLE_H
 #define ALTO_ORACLE_H
@@ -99,6 +104,13 @@ void orc_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, 
                            size_t bc);
 void orc_hint0(uint64_t *hint, size_t stride, const uint64_t *a, size_t npolys, size_t n, const uint8_t *db_col,
                size_t ld, size_t col0, size_t ncols);
+
+/* Offline packing precomputation (InspiRING) of one output part, and its NTT helpers. */
+void orc_ntt(uint64_t *a, size_t n, uint64_t psi);
+size_t orc_auto_slot(size_t t, size_t j, size_t n);
+int orc_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
+                          const uint64_t *hint, size_t stride, size_t col0, const uint64_t *w,
+                          const uint64_t *v, size_t n, size_t tg, uint64_t psi);
 
 /* Forward negacyclic NTT under the same slot convention, used only to pin orc_intt. */
 void orc_ntt_naive(uint64_t *out, const uint64_t *in, size_t n, uint64_t psi);

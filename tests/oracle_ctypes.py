@@ -72,6 +72,11 @@ def oracle() -> C.CDLL:
         lib.orc_intt_naive.argtypes = [P, P, SZ, C.c_uint64]
         lib.orc_powmod.restype = C.c_uint64
         lib.orc_powmod.argtypes = [C.c_uint64, C.c_uint64]
+        lib.orc_ntt.argtypes = [P, SZ, C.c_uint64]
+        lib.orc_auto_slot.restype = SZ
+        lib.orc_auto_slot.argtypes = [SZ, SZ, SZ]
+        lib.orc_inspiring_offline.restype = C.c_int
+        lib.orc_inspiring_offline.argtypes = [P, P, P, P, P, SZ, SZ, P, P, SZ, SZ, C.c_uint64]
         _oracle = lib
     return _oracle
 
@@ -227,3 +232,33 @@ def oracle_hint0(a: np.ndarray, db_col: np.ndarray, ld: int, n: int, col0: int =
     out = np.zeros((n, ncols), np.uint64)
     oracle().orc_hint0(out.ctypes.data, ncols, a.ctypes.data, npolys, n, db_col.ctypes.data, ld, col0, ncols)
     return out
+
+
+def oracle_offline(hint: np.ndarray, part: int, w: np.ndarray, v: np.ndarray, n: int, tg: int, psi: int):
+    """orc_inspiring_offline for output part `part` of an (n, cols) hint: (bold_t, bold_t_bar,
+    bold_t_hat, a_hat) as uint64 arrays."""
+    hint = np.ascontiguousarray(hint, dtype=np.uint64)
+    w = np.ascontiguousarray(w, dtype=np.uint64)
+    v = np.ascontiguousarray(v, dtype=np.uint64)
+    m = n // 2 - 1
+    bt = np.zeros(m * tg * n, np.uint64)
+    bb = np.zeros(m * tg * n, np.uint64)
+    bh = np.zeros(tg * n, np.uint64)
+    ah = np.zeros(n, np.uint64)
+    rc = oracle().orc_inspiring_offline(bt.ctypes.data, bb.ctypes.data, bh.ctypes.data, ah.ctypes.data,
+                                        hint.ctypes.data, hint.shape[1], part * n, w.ctypes.data, v.ctypes.data,
+                                        n, tg, psi)
+    assert rc == 0
+    return bt, bb, bh, ah
+
+
+def oracle_pack_build(bold_t, bold_t_bar, tables, gen_pows, num_iter: int, tg: int, n: int) -> np.ndarray:
+    """orc_pack_build: the packing matrix (int8, 4 tg n rho n) of rho parts."""
+    rho = len(bold_t)
+    ub = np.zeros(4 * tg * n * rho * n, np.int8)
+    p, pb = _ptrs(list(bold_t)), _ptrs(list(bold_t_bar))
+    tables = np.ascontiguousarray(tables, dtype=np.uint32)
+    gen_pows = np.ascontiguousarray(gen_pows, dtype=np.uint32)
+    oracle().orc_pack_build(ub.ctypes.data, C.addressof(p), C.addressof(pb), tables.ctypes.data, gen_pows.ctypes.data,
+                            num_iter, tg, n, rho)
+    return ub
