@@ -46,6 +46,13 @@ void set_err(const char *fmt, ...) {
     }())
 
 uint64_t host_mulmod(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) % kQ); }
+uint64_t host_powmod(uint64_t b, uint64_t e) {
+    uint64_t r = 1;
+    b %= kQ;
+    for (; e; e >>= 1, b = host_mulmod(b, b))
+        if (e & 1) r = host_mulmod(r, b);
+    return r;
+}
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
@@ -2017,3 +2024,6 @@ int alto_context_timings(void *c, double *ms, int count) {
 int alto_last_timings(void *p, double *ms, int count) { return p ? alto_context_timings(((Engine *)p)->ctx0, ms, count) : 0; }
 
 }  // extern "C"
+
+// the offline packing precomputation on the device (alto_inspiring_offline)
+#include "alto_offline.hpp"

@@ -81,6 +81,25 @@ int alto_engine_hint0(void *e, const uint64_t *a, size_t npolys, uint64_t *hint)
 int alto_hint0(const uint8_t *db_col, int on_device, size_t ld, size_t rows, size_t cols, size_t n, const uint64_t *a,
                size_t npolys, uint64_t *hint);
 
+/* No counterpart in the reference's C ABI: the offline packing precomputation of the
+ * reference's setup (full_packing_with_preprocessing_offline with prep_pack_many_lwes and
+ * precompute_aggregates_ntt, src/packing.rs:910-974, 2260-2359, called from
+ * perform_offline_precomputation_simplepir, src/server.rs:947-1021) for `parts` output parts on
+ * the device.  hint: n rows of hint_cols values (the offline hint, row z = coefficient z, as
+ * alto_hint0 writes it); part o takes columns [o n, (o + 1) n).  w, v: tg x n NTT slots of the
+ * two packing keys' masks.  Outputs (host memory, part-major, u64 values < Q): bold_t and
+ * bold_t_bar ((n/2 - 1) tg n per part, [(i tg + k) n + slot], what sw_pack_build /
+ * alto_pack_build read), bold_t_hat (tg n per part: the tail keys `that`), a_hat (n per
+ * part: the packed ciphertext's first row, coefficients).  The NTT is the engine's (slot j =
+ * evaluation at psi^(2 bitrev(j) + 1), psi the smallest primitive 2n-th root of unity mod Q,
+ * so the inverse table is psi^-bitrev(k) / 2); the gadget digits are base 2^ceil(32 / tg),
+ * least significant first.  n a power of two in [32, 4096], parts n <= hint_cols, tg <= 32
+ * with (1 + tg) n words within a workgroup's LDS.  Returns 0, or -1 (alto_last_error says
+ * why). */
+int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
+                           const uint64_t *hint, size_t hint_cols, size_t parts, const uint64_t *w,
+                           const uint64_t *v, size_t n, size_t tg);
+
 /* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
  * Returns nullptr for the configurations the reference rejects (Q != 4294955009,
  * n == 0 or n > 4096 or n % 16, cols % n, rows % 64, cols % 32, threads <= 0,
