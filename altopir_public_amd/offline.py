@@ -47,13 +47,15 @@ def prefix_from_a_hat(a_hat: np.ndarray, n: int) -> np.ndarray:
     return buf
 
 
-def device_hint(params: cl.Params, db_col: np.ndarray, ld: int | None = None) -> np.ndarray:
-    """The hint (n x cols) computed on the device from a column-major database (alto_hint0)."""
+def device_hint(params: cl.Params, db_col, ld: int | None = None, on_device: bool = False) -> np.ndarray:
+    """The hint (n x cols) computed on the device from a column-major database (alto_hint0):
+    db_col a host array, or with on_device the address of the database in device memory."""
     ld = params.rows if ld is None else ld
     a = np.ascontiguousarray(cl.hint_query_polys(params), dtype=np.uint64)
     out = np.zeros((params.n, params.cols), np.uint64)
     lib = _lib.load()
-    rc = lib.alto_hint0(db_col.ctypes.data, 0, ld, params.rows, params.cols, params.n, a.ctypes.data, a.shape[0],
+    ptr = db_col if on_device else db_col.ctypes.data
+    rc = lib.alto_hint0(ptr, 1 if on_device else 0, ld, params.rows, params.cols, params.n, a.ctypes.data, a.shape[0],
                         out.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"alto_hint0 failed ({_lib.last_error()})")
@@ -98,3 +100,67 @@ def server_inputs(params: cl.Params, db_col: np.ndarray, parts: list[tuple], ub:
                                  that=[np.ascontiguousarray(p[2], dtype=np.uint64) for p in parts],
                                  inv_table=workload.inverse_table(n), part_len=workload.part_len_for(n),
                                  row1_off=18 * n)
+
+
+def main(argv=None) -> int:
+    """Times the server's setup on the device for a database of --records x --record-bytes
+    uniform bytes generated in device memory (seed 0), then answers --queries client queries
+    with the HIP engine and checks that each decrypts to its record.  Prints one JSON line."""
+    import argparse
+    import json
+    import time
+
+    import torch
+
+    from . import synthetic
+    from .engine import Engine
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--records", type=int, default=32768)
+    ap.add_argument("--record-bytes", type=int, default=32768)
+    ap.add_argument("--queries", type=int, default=3)
+    args = ap.parse_args(argv)
+    p = cl.Params(args.records, args.record_bytes)
+    torch.cuda.set_device(0)
+    t = {}
+    d_db = synthetic.uniform_bytes_device(0, p.rows * p.cols)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    hint = device_hint(p, d_db.data_ptr(), on_device=True)
+    t["hint_s"] = time.time() - t0
+    t0 = time.time()
+    parts = device_offline(p, hint)
+    t["packing_precomputation_s"] = time.time() - t0
+    t0 = time.time()
+    inp = server_inputs(p, None, parts, device=True)
+    t["packing_matrix_s"] = time.time() - t0
+    del parts, hint
+    t0 = time.time()
+    eng = Engine.from_device_db(d_db.data_ptr(), inp, threads=16)
+    t["engine_s"] = time.time() - t0
+    t["setup_total_s"] = sum(t.values())
+    idx = [int(i) for i in np.random.default_rng(3).integers(0, p.rows, args.queries)]
+    clients, reqs = [], []
+    t0 = time.time()
+    for k, i in enumerate(idx):
+        c, f = cl.new_request(p, i, seed=50 + k)
+        clients.append(c)
+        reqs.append(f)
+    t["client_queries_s"] = time.time() - t0
+    out = np.zeros((len(reqs), p.response_len), np.uint8)
+    st = np.full(len(reqs), -1, np.int32)
+    answered, _ = eng.answer(reqs, out, st)
+    ok = answered == len(idx)
+    t0 = time.time()
+    for k, i in enumerate(idx):
+        rec = d_db.view(p.cols, p.rows)[:, i].cpu().numpy().tobytes()
+        ok = ok and cl.decrypt_response(p, clients[k], out[k]) == rec
+    t["client_decrypt_s"] = time.time() - t0
+    eng.close()
+    print(json.dumps({"records": p.rows, "record_bytes": p.cols, "n": p.n, "queries": len(idx),
+                      "decrypted_to_the_records": bool(ok), **{k: round(v, 3) for k, v in t.items()}}), flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
