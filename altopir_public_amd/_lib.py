@@ -21,6 +21,7 @@ SYMBOLS = (
     "alto_answer_gather", "alto_context_timings", "alto_context_stage_totals", "alto_stage_totals",
     "alto_context_set_graphs", "alto_context_graphs", "alto_set_fusion", "alto_fused", "alto_set_graph_timing",
     "matMulVecPacked", "matMulVecPacked2", "matMulVecPacked4", "matMulVecPacked8", "alto_matmul_vec_packed",
+    "alto_fast_batched_dot_product",
     # serving loop (bound in serve.py)
     "alto_server_new", "alto_server_submit", "alto_server_poll", "alto_server_release", "alto_server_contexts",
     "alto_server_get_stats", "alto_server_error", "alto_server_free", "alto_server_replay", "alto_server_set_graphs",
@@ -58,6 +59,8 @@ def load() -> C.CDLL:
     lib.alto_engine_hint0.argtypes = [P, P, SZ, P]
     lib.alto_hint0.restype = C.c_int
     lib.alto_hint0.argtypes = [P, C.c_int, SZ, SZ, SZ, SZ, P, SZ, P]
+    lib.alto_fast_batched_dot_product.restype = C.c_int
+    lib.alto_fast_batched_dot_product.argtypes = [P, SZ, P, SZ, SZ, P, C.c_int, SZ, SZ, SZ, C.c_uint64, C.c_uint64]
     lib.alto_inspiring_offline.restype = C.c_int
     lib.alto_inspiring_offline.argtypes = [P, P, P, P, P, SZ, SZ, P, P, SZ, SZ]
     lib.alto_pack_build.restype = C.c_int

@@ -301,3 +301,121 @@ extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, ui
     if (st) (void)hipStreamDestroy(st);
     return ok ? 0 : -1;
 }
+
+// ------------------------------------------------------------------------------------------
+// The two-modulus batched dot product of the reference's SimplePIR server path
+// (fast_batched_dot_product_avx512, src/kernel.rs:14-192, called from src/server.rs:593-618 and
+// :1446-1547), off the engine path: for batch row r < K and column j,
+//   lo = (sum_k (a[r][k] & 0xffffffff) * b[j][k] mod 2^64) mod m0
+//   hi = (sum_k (a[r][k] >> 32)        * b[j][k] mod 2^64) mod m1
+//   c[r][j] = (c[r][j] + crt(lo, hi)) mod m0 m1,   crt(lo, hi) = lo + m0 ((hi - lo) m0^-1 mod m1)
+// (a CRT-packed: a = (A mod m0) | (A mod m1) << 32 gives c + sum_k A b mod m0 m1).  b is
+// b_cols columns of b_rows elements of 1, 2 or 4 bytes (b_t[j * b_rows + k]).  One workgroup per
+// column j, every batch row at once: 256 threads split k, wrapping u64 partial sums, a tree sum.
+namespace offline {
+
+template <typename T>
+__global__ __launch_bounds__(256) void batched_dot_kernel(uint64_t *__restrict__ c, size_t c_stride,
+                                                          const uint64_t *__restrict__ a, size_t a_stride,
+                                                          const T *__restrict__ b, size_t b_rows, int K, uint64_t m0,
+                                                          uint64_t m1, uint64_t m0_inv) {
+    constexpr int kMaxK = 8;
+    __shared__ uint64_t red[2 * kMaxK][256];
+    const size_t j = blockIdx.x;
+    const T *bj = b + j * b_rows;
+    for (int r0 = 0; r0 < K; r0 += kMaxK) {
+        const int nk = min(kMaxK, K - r0);
+        uint64_t lo[kMaxK] = {}, hi[kMaxK] = {};
+        for (size_t k = threadIdx.x; k < b_rows; k += blockDim.x) {
+            const uint64_t bv = (uint64_t)bj[k];
+            for (int r = 0; r < nk; r++) {
+                const uint64_t av = a[(size_t)(r0 + r) * a_stride + k];
+                lo[r] += (av & 0xffffffffull) * bv;
+                hi[r] += (av >> 32) * bv;
+            }
+        }
+        for (int r = 0; r < nk; r++) {
+            red[2 * r][threadIdx.x] = lo[r];
+            red[2 * r + 1][threadIdx.x] = hi[r];
+        }
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+                for (int r = 0; r < 2 * nk; r++) red[r][threadIdx.x] += red[r][threadIdx.x + s];
+            __syncthreads();
+        }
+        if ((int)threadIdx.x < nk) {
+            const int r = threadIdx.x;
+            const uint64_t l = red[2 * r][0] % m0, h = red[2 * r + 1][0] % m1;
+            const uint64_t d = (h + m1 - l % m1) % m1;
+            const uint64_t t = (uint64_t)(((unsigned __int128)d * m0_inv) % m1);
+            const unsigned __int128 M = (unsigned __int128)m0 * m1;
+            const unsigned __int128 res = (unsigned __int128)l + (unsigned __int128)m0 * t;  // < M
+            uint64_t *cr = c + (size_t)(r0 + r) * c_stride + j;
+            *cr = (uint64_t)(((unsigned __int128)*cr + res) % M);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace offline
+
+extern "C" int alto_fast_batched_dot_product(uint64_t *c, size_t c_stride, const uint64_t *a, size_t a_stride,
+                                             size_t a_elems, const void *b_t, int b_bytes, size_t b_rows,
+                                             size_t b_cols, size_t K, uint64_t m0, uint64_t m1) {
+    using namespace offline;
+    const unsigned __int128 M = (unsigned __int128)m0 * m1;
+    if (!c || !a || !b_t || !K || !b_cols || a_elems != b_rows || !b_rows || m0 < 2 || m1 < 2 ||
+        M >> 64 || c_stride < b_cols || a_stride < a_elems || (b_bytes != 1 && b_bytes != 2 && b_bytes != 4) ||
+        b_cols > 0x7fffffffu) {
+        set_err("alto_fast_batched_dot_product: bad arguments (a_elems == b_rows, m0 m1 < 2^64, b of 1, 2 or 4 bytes)");
+        return -1;
+    }
+    // m0^-1 mod m1 (extended Euclid); the moduli must be coprime
+    long long t0 = 0, t1 = 1;
+    unsigned long long r0 = m1, r1 = m0 % m1;
+    while (r1) {
+        const unsigned long long q = r0 / r1, r2 = r0 - q * r1;
+        const long long t2 = t0 - (long long)q * t1;
+        r0 = r1;
+        r1 = r2;
+        t0 = t1;
+        t1 = t2;
+    }
+    if (r0 != 1) {
+        set_err("alto_fast_batched_dot_product: the moduli are not coprime");
+        return -1;
+    }
+    const uint64_t m0_inv = (uint64_t)((t0 % (long long)m1 + (long long)m1) % (long long)m1);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return -1;
+    }
+    uint64_t *dc = nullptr, *da = nullptr;
+    void *db = nullptr;
+    hipStream_t st = nullptr;
+    const size_t cb = K * c_stride * 8, ab = K * a_stride * 8, bb = b_cols * b_rows * (size_t)b_bytes;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && HIP_OK(hipMalloc(&dc, cb)) &&
+              HIP_OK(hipMalloc(&da, ab)) && HIP_OK(hipMalloc(&db, bb)) &&
+              HIP_OK(hipMemcpyAsync(dc, c, cb, hipMemcpyHostToDevice, st)) &&
+              HIP_OK(hipMemcpyAsync(da, a, ab, hipMemcpyHostToDevice, st)) &&
+              HIP_OK(hipMemcpyAsync(db, b_t, bb, hipMemcpyHostToDevice, st));
+    if (ok) {
+        if (b_bytes == 1)
+            hipLaunchKernelGGL(batched_dot_kernel<uint8_t>, dim3((unsigned)b_cols), dim3(256), 0, st, dc, c_stride,
+                               (const uint64_t *)da, a_stride, (const uint8_t *)db, b_rows, (int)K, m0, m1, m0_inv);
+        else if (b_bytes == 2)
+            hipLaunchKernelGGL(batched_dot_kernel<uint16_t>, dim3((unsigned)b_cols), dim3(256), 0, st, dc, c_stride,
+                               (const uint64_t *)da, a_stride, (const uint16_t *)db, b_rows, (int)K, m0, m1, m0_inv);
+        else
+            hipLaunchKernelGGL(batched_dot_kernel<uint32_t>, dim3((unsigned)b_cols), dim3(256), 0, st, dc, c_stride,
+                               (const uint64_t *)da, a_stride, (const uint32_t *)db, b_rows, (int)K, m0, m1, m0_inv);
+        ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(c, dc, cb, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipStreamSynchronize(st));
+    }
+    for (void *p : {(void *)dc, (void *)da, db})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}

@@ -139,6 +139,19 @@ void matMulVecPacked2(uint32_t *out, const uint32_t *a, const uint32_t *b_full, 
 void matMulVecPacked4(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols);
 void matMulVecPacked8(uint32_t *out, const uint32_t *a, const uint32_t *b_full, size_t aRows, size_t aCols);
 
+/* replaces fast_batched_dot_product_avx512<K, T> (src/kernel.rs:14-22, 143-151), the
+ * two-modulus (CRT) product of the reference's SimplePIR server path, off the engine path
+ * (callers src/server.rs:593-618, 1446-1547).  The reference passes spiral-rs Params; here the
+ * two moduli come explicitly (m0 m1 < 2^64, coprime) and the strides of a and c (the
+ * reference's a.len() / K and c.len() / K).  For r < K, j < b_cols:
+ *   c[r c_stride + j] = (c + crt(sum_k lo(a[r a_stride + k]) b[j b_rows + k] mod 2^64 mod m0,
+ *                                sum_k hi(a[...]) b[...] mod 2^64 mod m1)) mod m0 m1,
+ * lo / hi the low and high 32 bits, b elements of b_bytes = 1, 2 or 4 bytes (the reference's T),
+ * a_elems == b_rows.  Host buffers; computed on the device.  0, or -1 (alto_last_error). */
+int alto_fast_batched_dot_product(uint64_t *c, size_t c_stride, const uint64_t *a, size_t a_stride, size_t a_elems,
+                                  const void *b_t, int b_bytes, size_t b_rows, size_t b_cols, size_t K, uint64_t m0,
+                                  uint64_t m1);
+
 /* ---------------------------------------------------------------- Extensions -- */
 
 /* The packed product for any number of columns of b; 0, or -1 (bad arguments, no device,

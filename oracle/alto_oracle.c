@@ -587,3 +587,38 @@ int orc_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold
     free(at); free(ev); free(r); free(rb); free(c); free(gp);
     return 0;
 }
+
+/* The two-modulus batched dot product of src/kernel.rs:143-192 (the reference's scalar form):
+ * for batch row r < K and column j, the wrapping u64 sums of (a & 0xffffffff) b and (a >> 32) b
+ * over k, reduced mod m0 and m1 (barrett_coeff_u64), composed by CRT into Z_{m0 m1}
+ * (crt_compose_2) and added to c mod m0 m1 (barrett_u64).  b elements of b_bytes bytes. */
+void orc_fast_batched_dot_product(uint64_t *c, size_t c_stride, const uint64_t *a, size_t a_stride,
+                                  size_t a_elems, const void *b_t, int b_bytes, size_t b_rows, size_t b_cols,
+                                  size_t K, uint64_t m0, uint64_t m1) {
+    uint64_t inv = 0;  /* m0^-1 mod m1 (extended Euclid: m1 need not be prime) */
+    {
+        __int128 t0 = 0, t1 = 1, r0 = m1, r1 = m0 % m1;
+        while (r1) {
+            __int128 q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
+            r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+        }
+        inv = (uint64_t)(((t0 % (__int128)m1) + m1) % m1);
+    }
+    const u128 M = (u128)m0 * m1;
+    for (size_t r = 0; r < K; r++)
+        for (size_t j = 0; j < b_cols; j++) {
+            uint64_t lo = 0, hi = 0;
+            for (size_t k = 0; k < a_elems; k++) {
+                const size_t bi = j * b_rows + k;
+                const uint64_t bv = b_bytes == 1 ? ((const uint8_t *)b_t)[bi]
+                                  : b_bytes == 2 ? ((const uint16_t *)b_t)[bi] : ((const uint32_t *)b_t)[bi];
+                const uint64_t av = a[r * a_stride + k];
+                lo += (av & 0xffffffffull) * bv;
+                hi += (av >> 32) * bv;
+            }
+            const uint64_t l = lo % m0, h = hi % m1;
+            const uint64_t t = (uint64_t)((u128)((h + m1 - l % m1) % m1) * inv % m1);
+            const u128 res = (u128)l + (u128)m0 * t;
+            c[r * c_stride + j] = (uint64_t)(((u128)c[r * c_stride + j] + res) % M);
+        }
+}

@@ -35,6 +35,8 @@
  *                     polynomials, then the fold x^n = -1), and the closed forms for the query
  *                     polynomials 1 and x (sums of record bytes, shifted with one negated term)
  *   orc_ntt           orc_ntt_naive (slot evaluation), and orc_intt inverts it
+ *   orc_fast_batched_dot_product  the closed form c + sum_k A b mod m0 m1 for CRT-packed
+ *                     A (big integers), and the u64 wrap of the partial sums
  *   orc_inspiring_offline  the aggregates against the reference's direct form (sums of
  *                     monomial products, src/packing.rs:2179-2258, evaluated term by term);
  *                     the whole precomputation by decryption: responses the engine computes
@@ -111,6 +113,10 @@ size_t orc_auto_slot(size_t t, size_t j, size_t n);
 int orc_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
                           const uint64_t *hint, size_t stride, size_t col0, const uint64_t *w,
                           const uint64_t *v, size_t n, size_t tg, uint64_t psi);
+
+void orc_fast_batched_dot_product(uint64_t *c, size_t c_stride, const uint64_t *a, size_t a_stride,
+                                  size_t a_elems, const void *b_t, int b_bytes, size_t b_rows, size_t b_cols,
+                                  size_t K, uint64_t m0, uint64_t m1);
 
 /* Forward negacyclic NTT under the same slot convention, used only to pin orc_intt. */
 void orc_ntt_naive(uint64_t *out, const uint64_t *in, size_t n, uint64_t psi);

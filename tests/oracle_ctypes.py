@@ -75,6 +75,8 @@ def oracle() -> C.CDLL:
         lib.orc_ntt.argtypes = [P, SZ, C.c_uint64]
         lib.orc_auto_slot.restype = SZ
         lib.orc_auto_slot.argtypes = [SZ, SZ, SZ]
+        lib.orc_fast_batched_dot_product.argtypes = [P, SZ, P, SZ, SZ, P, C.c_int, SZ, SZ, SZ, C.c_uint64,
+                                                     C.c_uint64]
         lib.orc_inspiring_offline.restype = C.c_int
         lib.orc_inspiring_offline.argtypes = [P, P, P, P, P, SZ, SZ, P, P, SZ, SZ, C.c_uint64]
         _oracle = lib

@@ -546,3 +546,28 @@ def test_engine_from_records_matches_the_oracle(hip_device, rows, cols, n, ld_pa
     r_o = oc.oracle_answer(inp, reqs, out_o, st_o)
     assert answered == r_o == B and np.array_equal(st, st_o)
     assert np.array_equal(out, out_o), "responses of the record-ingested engine differ from the oracle"
+
+
+@pytest.mark.parametrize("b_bytes,K,a_elems,b_cols", [(1, 1, 4096, 300), (2, 3, 1000, 64), (4, 2, 777, 33),
+                                                      (2, 11, 513, 17)])
+def test_fast_batched_dot_product_matches_the_oracle(hip_device, b_bytes, K, a_elems, b_cols):
+    # src/kernel.rs:14-192 with explicit moduli: the device's u64 partial sums wrap like the
+    # reference's, then the CRT composition; full-range operands (sums past 2^64 included)
+    from altopir_public_amd import _lib
+
+    m0, m1 = 4294967291, 4294967279
+    rng = np.random.default_rng(b_bytes * 100 + K)
+    a = rng.integers(0, 1 << 63, (K, a_elems), dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, (K, a_elems),
+                                                                                               dtype=np.uint64)
+    b = rng.integers(0, 1 << (8 * b_bytes), (b_cols, a_elems), dtype=np.uint64).astype(f"<u{b_bytes}")
+    c = rng.integers(0, m0 * m1, (K, b_cols + 3), dtype=np.uint64)
+    want = c.copy()
+    oc.oracle().orc_fast_batched_dot_product(want.ctypes.data, b_cols + 3, a.ctypes.data, a_elems, a_elems,
+                                             b.ctypes.data, b_bytes, a_elems, b_cols, K, m0, m1)
+    got = c.copy()
+    assert _lib.load().alto_fast_batched_dot_product(got.ctypes.data, b_cols + 3, a.ctypes.data, a_elems, a_elems,
+                                                     b.ctypes.data, b_bytes, a_elems, b_cols, K, m0, m1) == 0
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[:, b_cols:], c[:, b_cols:])  # columns past b_cols untouched
+    assert _lib.load().alto_fast_batched_dot_product(got.ctypes.data, b_cols + 3, a.ctypes.data, a_elems, a_elems,
+                                                     b.ctypes.data, 3, a_elems, b_cols, K, m0, m1) == -1

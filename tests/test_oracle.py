@@ -520,3 +520,43 @@ def test_hint_of_the_polynomials_one_and_x():
     want[1:] = d[:, :, :-1].sum(axis=1).T
     want[0] = -d[:, :, -1].sum(axis=1)
     assert np.array_equal(oc.oracle_hint0(x, db_col, n * npolys, n), (want % Q).astype(np.uint64))
+
+
+def _crt_case(K, a_elems, b_cols, b_bytes, seed, m0=4294967291, m1=4294967279):
+    rng = np.random.default_rng(seed)
+    A = [[int(x) for x in rng.integers(0, 1 << 62, a_elems)] for _ in range(K)]
+    A = [[x % (m0 * m1) for x in row] for row in A]
+    a = np.array([[(x % m0) | ((x % m1) << 32) for x in row] for row in A], np.uint64)
+    # 4-byte values below 2^26, so that the sums stay below 2^64 (the closed form has no wrap)
+    b = rng.integers(0, 1 << min(8 * b_bytes, 26), (b_cols, a_elems), dtype=np.uint64).astype(f"<u{b_bytes}")
+    c = rng.integers(0, m0 * m1, (K, b_cols), dtype=np.uint64)
+    return A, a, b, c
+
+
+@pytest.mark.parametrize("b_bytes,K,a_elems", [(1, 1, 64), (2, 3, 100), (4, 2, 37)])
+def test_fast_batched_dot_product_is_the_crt_closed_form(b_bytes, K, a_elems):
+    # src/kernel.rs:143-192 for CRT-packed a (a = A mod m0 | A mod m1 << 32) and sums below 2^64:
+    # c + sum_k A b mod m0 m1, in big integers
+    m0, m1 = 4294967291, 4294967279
+    b_cols = 5
+    A, a, b, c = _crt_case(K, a_elems, b_cols, b_bytes, 11 + b_bytes)
+    want = [[(int(c[r, j]) + sum(A[r][k] * int(b[j, k]) for k in range(a_elems))) % (m0 * m1) for j in range(b_cols)]
+            for r in range(K)]
+    got = c.copy()
+    oc.oracle().orc_fast_batched_dot_product(got.ctypes.data, b_cols, a.ctypes.data, a_elems, a_elems, b.ctypes.data,
+                                             b_bytes, a_elems, b_cols, K, m0, m1)
+    assert [[int(x) for x in row] for row in got] == want
+
+
+def test_fast_batched_dot_product_wraps_its_partial_sums():
+    # the partial sums wrap mod 2^64 before the reduction (wrapping_add, src/kernel.rs:180-181)
+    m0, m1, a_elems = 1000003, 998244353, 5000
+    a = np.full((1, a_elems), 0xFFFFFFFF | (0xFFFFFFFF << 32), np.uint64)
+    b = np.full((1, a_elems), 0xFFFFFFFF, "<u4")
+    c = np.zeros((1, 1), np.uint64)
+    oc.oracle().orc_fast_batched_dot_product(c.ctypes.data, 1, a.ctypes.data, a_elems, a_elems, b.ctypes.data, 4,
+                                             a_elems, 1, 1, m0, m1)
+    s = (a_elems * 0xFFFFFFFF * 0xFFFFFFFF) % (1 << 64)
+    lo, hi = s % m0, s % m1
+    x = next(v for v in range(lo, m0 * m1, m0) if v % m1 == hi)
+    assert int(c[0, 0]) == x
