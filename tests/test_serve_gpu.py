@@ -235,6 +235,21 @@ def test_load_generator_end_to_end(hip_device, capsys):
     assert r["p99_ms"] >= r["p50_ms"] > 0
 
 
+def test_load_generator_with_scheme_queries(hip_device, capsys):
+    # the same with the databases set up as a server does and the scheme's client queries: the
+    # sampled responses decrypt to their records
+    import json
+
+    from altopir_public_amd import loadgen
+
+    assert loadgen.main(["--gib", "0.015625", "0.03125", "--record-bytes", "4096", "--rates", "200",
+                         "--duration", "1", "--corrupt", "0.05", "--pool", "3", "--max-batch", "16",
+                         "--slots", "64", "--scheme", "--decrypt-sample", "24"]) == 0
+    r = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")][0]
+    assert r["rejected_as_expected"] and r["sent"] == 200
+    assert r["decrypted_sample"] == 24 and r["sample_decrypts_to_the_records"] is True
+
+
 def test_release_frees_only_a_leased_slot(hip_device, two_dbs):
     # a slot goes back to the pool once: a second release, or the release of a slot that was
     # never handed out with a response, is refused (two requests would otherwise share a slot)
