@@ -66,7 +66,7 @@ def device_offline(params: cl.Params, hint: np.ndarray) -> list[tuple]:
     """The packing precomputation of every output part on the device
     (alto_inspiring_offline): a list of (bold_t, bold_t_bar, bold_t_hat, a_hat) per part."""
     n, tg, O = params.n, params.tg, params.outputs
-    w, v = cl.masks(params)
+    w, v = (np.ascontiguousarray(x, dtype=np.uint64) for x in cl.masks(params))
     hint = np.ascontiguousarray(hint, dtype=np.uint64)
     m = n // 2 - 1
     bt = np.zeros((O, m * tg * n), np.uint64)
@@ -75,8 +75,7 @@ def device_offline(params: cl.Params, hint: np.ndarray) -> list[tuple]:
     ah = np.zeros((O, n), np.uint64)
     lib = _lib.load()
     rc = lib.alto_inspiring_offline(bt.ctypes.data, bb.ctypes.data, bh.ctypes.data, ah.ctypes.data,
-                                    hint.ctypes.data, params.cols, O, np.ascontiguousarray(w).ctypes.data,
-                                    np.ascontiguousarray(v).ctypes.data, n, tg)
+                                    hint.ctypes.data, params.cols, O, w.ctypes.data, v.ctypes.data, n, tg)
     if rc != 0:
         raise RuntimeError(f"alto_inspiring_offline failed ({_lib.last_error()})")
     return [(bt[o], bb[o], bh[o], ah[o]) for o in range(O)]
