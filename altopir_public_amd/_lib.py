@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libaltopir_hip.so"
 # every entry point declared in include/altopir_engine.h
 SYMBOLS = (
     "sw_amx_init", "sw_alloc_zeroed", "sw_alloc", "sw_free", "sw_tile_db", "sw_pack_build", "alto_pack_build",
-    "alto_engine_hint0", "alto_hint0", "alto_inspiring_offline",
+    "alto_engine_hint0", "alto_hint0", "alto_inspiring_offline", "alto_scheme_setup",
     "alto_engine_new", "alto_engine_free", "alto_request_len", "alto_response_len", "alto_answer",
     "alto_answer_device", "alto_answer_device_noalloc", "alto_engine_device_bytes", "alto_last_error", "alto_last_timings",
     "alto_variant_count", "alto_variant_name", "alto_tune_set", "alto_tune_get", "alto_set_graphs",
@@ -61,6 +61,8 @@ def load() -> C.CDLL:
     lib.alto_hint0.argtypes = [P, C.c_int, SZ, SZ, SZ, SZ, P, SZ, P]
     lib.alto_fast_batched_dot_product.restype = C.c_int
     lib.alto_fast_batched_dot_product.argtypes = [P, SZ, P, SZ, SZ, P, C.c_int, SZ, SZ, SZ, C.c_uint64, C.c_uint64]
+    lib.alto_scheme_setup.restype = C.c_int
+    lib.alto_scheme_setup.argtypes = [P, P, P, P, C.c_int, SZ, SZ, SZ, P, SZ, P, P, SZ, SZ]
     lib.alto_inspiring_offline.restype = C.c_int
     lib.alto_inspiring_offline.argtypes = [P, P, P, P, P, SZ, SZ, P, P, SZ, SZ]
     lib.alto_pack_build.restype = C.c_int

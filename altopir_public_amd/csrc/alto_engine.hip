@@ -1464,17 +1464,15 @@ bool tile_db_columns(const uint8_t *db_col, bool on_device, size_t ld, size_t ro
 // `cols` columns on the current device: the n x rows expansion of the query polynomials as
 // limb rows (hint_limb_kernel), the scan contraction with the exact mod-Q store, and the
 // 128 S_z correction (hint_finish_kernel).
-bool hint_from_tiled(const int8_t *db, size_t rows, size_t cols, size_t n, const uint64_t *a, size_t npolys,
-                     uint64_t *hint) {
+// The hint into device memory (dout: n x cols u64), enqueued on st and waited for.
+bool hint_device(const int8_t *db, size_t rows, size_t cols, size_t n, const uint64_t *a, size_t npolys,
+                 uint64_t *dout, hipStream_t st) {
     const size_t C = cols, KC = rows / 64, MT = n / 4;
-    uint64_t *da = nullptr, *dout = nullptr;
+    uint64_t *da = nullptr;
     int8_t *qa = nullptr;
     unsigned long long *acc = nullptr, *ssum = nullptr;
-    hipStream_t st = nullptr;
-    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
-              HIP_OK(hipMalloc(&da, npolys * n * 8)) && HIP_OK(hipMalloc(&qa, MT * KC * kBlockBytes)) &&
-              HIP_OK(hipMalloc(&acc, n * C * 8)) && HIP_OK(hipMalloc(&ssum, n * 8)) &&
-              HIP_OK(hipMalloc(&dout, n * C * 8));
+    bool ok = HIP_OK(hipMalloc(&da, npolys * n * 8)) && HIP_OK(hipMalloc(&qa, MT * KC * kBlockBytes)) &&
+              HIP_OK(hipMalloc(&acc, n * C * 8)) && HIP_OK(hipMalloc(&ssum, n * 8));
     ok = ok && HIP_OK(hipMemcpyAsync(da, a, npolys * n * 8, hipMemcpyHostToDevice, st)) &&
          HIP_OK(hipMemsetAsync(acc, 0, n * C * 8, st)) && HIP_OK(hipMemsetAsync(ssum, 0, n * 8, st));
     if (ok) {
@@ -1491,13 +1489,38 @@ bool hint_from_tiled(const int8_t *db, size_t rows, size_t cols, size_t n, const
         v.scan(g, qa, db, st);
         hipLaunchKernelGGL(hint_finish_kernel, dim3((unsigned)((n * C + 255) / 256)), dim3(256), 0, st, dout,
                            (const unsigned long long *)acc, (const unsigned long long *)ssum, C, n * C);
-        ok = HIP_OK(hipGetLastError()) && HIP_OK(hipMemcpyAsync(hint, dout, n * C * 8, hipMemcpyDeviceToHost, st)) &&
-             HIP_OK(hipStreamSynchronize(st));
+        ok = HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(st));
     }
-    for (void *q : {(void *)da, (void *)dout, (void *)qa, (void *)acc, (void *)ssum})
+    for (void *q : {(void *)da, (void *)qa, (void *)acc, (void *)ssum})
         if (q) (void)hipFree(q);
+    return ok;
+}
+
+bool hint_from_tiled(const int8_t *db, size_t rows, size_t cols, size_t n, const uint64_t *a, size_t npolys,
+                     uint64_t *hint) {
+    uint64_t *dout = nullptr;
+    hipStream_t st = nullptr;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && HIP_OK(hipMalloc(&dout, n * cols * 8)) &&
+              hint_device(db, rows, cols, n, a, npolys, dout, st) &&
+              HIP_OK(hipMemcpyAsync(hint, dout, n * cols * 8, hipMemcpyDeviceToHost, st)) &&
+              HIP_OK(hipStreamSynchronize(st));
+    if (dout) (void)hipFree(dout);
     if (st) (void)hipStreamDestroy(st);
     return ok;
+}
+
+// The packing matrix (tile layout of sw_pack_build) from device bold_t / bold_t_bar (part-major,
+// per values a part), device tables (n x n) and generator powers; enqueued on st.
+bool pack_build_device(const uint64_t *dbt, const uint64_t *dbb, size_t per, const uint32_t *dtab,
+                       const uint32_t *dpow, size_t num_iter, size_t tg, size_t n, size_t cols, uint64_t qmod,
+                       int8_t *dub, hipStream_t st) {
+    const size_t lds = tg * n * 8;
+    if (!HIP_OK(hipFuncSetAttribute((const void *)pack_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds)))
+        return false;
+    hipLaunchKernelGGL(pack_build_kernel, dim3((unsigned)cols), dim3(256), lds, st, dbt, dbb, per, dtab, dpow,
+                       (int)num_iter, (int)tg, (int)n, (int)cols, qmod, dub);
+    return HIP_OK(hipGetLastError());
 }
 
 bool hint_shape_ok(const char *fn, size_t rows, size_t cols, size_t n, size_t npolys) {
@@ -1582,17 +1605,7 @@ int alto_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *c
              HIP_OK(hipMemcpyAsync(dbb + o * per, bold_t_bar[o], per * 8, hipMemcpyHostToDevice, st));
     ok = ok && HIP_OK(hipMemcpyAsync(dtab, tables, n * n * 4, hipMemcpyHostToDevice, st)) &&
          HIP_OK(hipMemcpyAsync(dpow, gen_pows, num_iter * 4, hipMemcpyHostToDevice, st));
-    if (ok) {
-        const size_t lds = K * 8;
-        ok = HIP_OK(hipFuncSetAttribute((const void *)pack_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-        if (ok) {
-            hipLaunchKernelGGL(pack_build_kernel, dim3((unsigned)cols), dim3(256), lds, st, (const uint64_t *)dbt,
-                               (const uint64_t *)dbb, per, (const uint32_t *)dtab, (const uint32_t *)dpow,
-                               (int)num_iter, (int)tg, (int)n, (int)cols, qmod, dub);
-            ok = HIP_OK(hipGetLastError());
-        }
-    }
+    ok = ok && pack_build_device(dbt, dbb, per, dtab, dpow, num_iter, tg, n, cols, qmod, dub, st);
     ok = ok && HIP_OK(hipMemcpyAsync(ub, dub, ubytes, hipMemcpyDeviceToHost, st)) && HIP_OK(hipStreamSynchronize(st));
     for (void *p : {(void *)dbt, (void *)dbb, (void *)dtab, (void *)dpow, (void *)dub})
         if (p) (void)hipFree(p);

@@ -210,24 +210,26 @@ __global__ __launch_bounds__(256) void finish_kernel(const uint32_t *__restrict_
 
 }  // namespace offline
 
-extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
-                                      const uint64_t *hint, size_t hint_cols, size_t parts, const uint64_t *w,
-                                      const uint64_t *v, size_t n, size_t tg) {
-    using namespace offline;
+namespace offline {
+
+bool shape_ok(const char *fn, size_t parts, size_t n, size_t tg, size_t hint_cols) {
+    const size_t lds_walk = (1 + tg) * n * 4;
+    if (!parts || !tg || tg > 32 || n < 32 || n > 4096 || (n & (n - 1)) || (kQ - 1) % (2 * n) ||
+        parts * n > hint_cols || lds_walk > kMaxLds) {
+        set_err("%s: unsupported shape (n a power of two in [32, 4096], parts n <= hint columns, (1 + tg) n words "
+                "of LDS)", fn);
+        return false;
+    }
+    return true;
+}
+
+// The precomputation of `parts` output parts from a hint in device memory (n x hint_cols u64),
+// outputs in device memory (dbt, dbb: parts x (n/2 - 1) tg n; dbh: parts x tg n; dah: parts x
+// n); enqueued on st and waited for.
+bool offline_device(const uint64_t *dhint, size_t hint_cols, size_t parts, const uint64_t *w, const uint64_t *v,
+                    size_t n, size_t tg, uint64_t *dbt, uint64_t *dbb, uint64_t *dbh, uint64_t *dah, hipStream_t st) {
     int logn = 0;
     while (((size_t)1 << logn) < n) logn++;
-    const size_t lds_walk = (1 + tg) * n * 4;
-    if (!bold_t || !bold_t_bar || !bold_t_hat || !a_hat || !hint || !w || !v || !parts || !tg || tg > 32 ||
-        n < 32 || n > 4096 || (n & (n - 1)) || (kQ - 1) % (2 * n) || parts * n > hint_cols || lds_walk > kMaxLds) {
-        set_err("alto_inspiring_offline: unsupported shape (n a power of two in [32, 4096], parts n <= hint columns, "
-                "(1 + tg) n words of LDS)");
-        return -1;
-    }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        set_err("no HIP device available");
-        return -1;
-    }
     // the engine's NTT: psi the smallest primitive 2n-th root of unity
     uint64_t psi = 0;
     for (uint64_t g = 2; !psi && g < (1u << 16); g++) {
@@ -235,7 +237,8 @@ extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, ui
         if (host_powmod(c, n) == kQ - 1) psi = c;
     }
     const uint64_t psi_inv = host_powmod(psi, kQ - 2), half_inv = (kQ + 1) / 2;
-    std::vector<uint32_t> fwd(n), inv(n), gp(n / 2), wv(2 * tg * n);
+    std::vector<uint32_t> tab(2 * n + n / 2 + 2 * tg * n);
+    uint32_t *fwd = tab.data(), *inv = fwd + n, *gp = inv + n, *wv = gp + n / 2;
     for (size_t k = 0; k < n; k++) {
         size_t b = 0;
         for (int i = 0; i < logn; i++) b |= ((k >> i) & 1) << (logn - 1 - i);
@@ -248,28 +251,18 @@ extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, ui
         wv[i] = (uint32_t)(w[i] % kQ);
         wv[tg * n + i] = (uint32_t)(v[i] % kQ);
     }
-    const size_t O = parts, nn = n * n, half = n / 2, per = (half - 1) * tg * n;
-    uint64_t *dhint = nullptr, *dbt = nullptr, *dbb = nullptr, *dbh = nullptr, *dah = nullptr;
+    const size_t O = parts, nn = n * n, half = n / 2, lds_walk = (1 + tg) * n * 4;
     uint32_t *dA = nullptr, *dB = nullptr, *dr = nullptr, *drb = nullptr, *dr0 = nullptr, *dtab = nullptr;
-    hipStream_t st = nullptr;
-    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
-              HIP_OK(hipMalloc(&dhint, n * hint_cols * 8)) && HIP_OK(hipMalloc(&dA, O * nn * 4)) &&
-              HIP_OK(hipMalloc(&dB, O * nn * 4)) && HIP_OK(hipMalloc(&dr, O * half * n * 4)) &&
-              HIP_OK(hipMalloc(&drb, O * half * n * 4)) && HIP_OK(hipMalloc(&dr0, O * n * 4)) &&
-              HIP_OK(hipMalloc(&dbt, O * per * 8)) && HIP_OK(hipMalloc(&dbb, O * per * 8)) &&
-              HIP_OK(hipMalloc(&dbh, O * tg * n * 8)) && HIP_OK(hipMalloc(&dah, O * n * 8)) &&
-              HIP_OK(hipMalloc(&dtab, (2 * n + n / 2 + 2 * tg * n) * 4));
+    bool ok = HIP_OK(hipMalloc(&dA, O * nn * 4)) && HIP_OK(hipMalloc(&dB, O * nn * 4)) &&
+              HIP_OK(hipMalloc(&dr, O * half * n * 4)) && HIP_OK(hipMalloc(&drb, O * half * n * 4)) &&
+              HIP_OK(hipMalloc(&dr0, O * n * 4)) && HIP_OK(hipMalloc(&dtab, tab.size() * 4)) &&
+              HIP_OK(hipMemcpyAsync(dtab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
     uint32_t *dfwd = dtab, *dinv = dtab + n, *dgp = dtab + 2 * n, *dw = dgp + n / 2, *dv = dw + tg * n;
-    ok = ok && HIP_OK(hipMemcpyAsync(dhint, hint, n * hint_cols * 8, hipMemcpyHostToDevice, st)) &&
-         HIP_OK(hipMemcpyAsync(dfwd, fwd.data(), n * 4, hipMemcpyHostToDevice, st)) &&
-         HIP_OK(hipMemcpyAsync(dinv, inv.data(), n * 4, hipMemcpyHostToDevice, st)) &&
-         HIP_OK(hipMemcpyAsync(dgp, gp.data(), n / 2 * 4, hipMemcpyHostToDevice, st)) &&
-         HIP_OK(hipMemcpyAsync(dw, wv.data(), 2 * tg * n * 4, hipMemcpyHostToDevice, st));
     if (ok) {
         const unsigned nt = (unsigned)(n / 32);
         // hint columns as rows (A: O n rows of n), a~ = NTT(sigma(row)) in place
-        hipLaunchKernelGGL(hint_transpose_kernel, dim3((unsigned)(O * n / 32), nt), dim3(256), 0, st,
-                           (const uint64_t *)dhint, hint_cols, dA, (int)n, (int)(O * n));
+        hipLaunchKernelGGL(hint_transpose_kernel, dim3((unsigned)(O * n / 32), nt), dim3(256), 0, st, dhint,
+                           hint_cols, dA, (int)n, (int)(O * n));
         hipLaunchKernelGGL(ntt_rows_kernel<true>, dim3((unsigned)(O * n)), dim3(256), n * 4, st, dA, (int)n,
                            (const uint32_t *)dfwd);
         // rows v of each part's a~ matrix (B), E = NTT(row_v) in place
@@ -278,8 +271,7 @@ extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, ui
         hipLaunchKernelGGL(ntt_rows_kernel<false>, dim3((unsigned)(O * n)), dim3(256), n * 4, st, dB, (int)n,
                            (const uint32_t *)dfwd);
         hipLaunchKernelGGL(gather_kernel, dim3((unsigned)half, (unsigned)O), dim3(256), 0, st, (const uint32_t *)dB,
-                           dr, drb, (const uint32_t *)dgp, (int)n, logn,
-                           (uint32_t)host_powmod(n, kQ - 2));
+                           dr, drb, (const uint32_t *)dgp, (int)n, logn, (uint32_t)host_powmod(n, kQ - 2));
         WalkArgs wa{dr, drb, dw, dfwd, dinv, dgp, dbt, dbb, dbh, dr0, (int)n, logn, (int)tg, (int)((32 + tg - 1) / tg)};
         ok = HIP_OK(hipFuncSetAttribute((const void *)walk_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds_walk));
@@ -288,15 +280,111 @@ extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, ui
             hipLaunchKernelGGL(finish_kernel, dim3((unsigned)O), dim3(256), n * 4, st, (const uint32_t *)dr0,
                                (const uint64_t *)dbh, (const uint32_t *)dv, (const uint32_t *)dinv, dah, (int)n,
                                (int)tg);
-            ok = HIP_OK(hipGetLastError());
+            ok = HIP_OK(hipGetLastError()) && HIP_OK(hipStreamSynchronize(st));
         }
     }
+    for (void *p : {(void *)dA, (void *)dB, (void *)dr, (void *)drb, (void *)dr0, (void *)dtab})
+        if (p) (void)hipFree(p);
+    return ok;
+}
+
+// Every automorphism's slot table, table (t - 1) / 2 for odd t < 2n (n x n u32)
+__global__ __launch_bounds__(256) void tables_kernel(uint32_t *__restrict__ tab, int n, int logn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n) return;
+    const uint32_t ti = (uint32_t)(i / n), s = (uint32_t)(i % n);
+    tab[i] = auto_slot(2 * ti + 1, s, logn);
+}
+
+}  // namespace offline
+
+extern "C" int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bold_t_hat, uint64_t *a_hat,
+                                      const uint64_t *hint, size_t hint_cols, size_t parts, const uint64_t *w,
+                                      const uint64_t *v, size_t n, size_t tg) {
+    using namespace offline;
+    if (!bold_t || !bold_t_bar || !bold_t_hat || !a_hat || !hint || !w || !v) {
+        set_err("alto_inspiring_offline: null argument");
+        return -1;
+    }
+    if (!shape_ok("alto_inspiring_offline", parts, n, tg, hint_cols)) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return -1;
+    }
+    const size_t O = parts, per = (n / 2 - 1) * tg * n;
+    uint64_t *dhint = nullptr, *dbt = nullptr, *dbb = nullptr, *dbh = nullptr, *dah = nullptr;
+    hipStream_t st = nullptr;
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+              HIP_OK(hipMalloc(&dhint, n * hint_cols * 8)) && HIP_OK(hipMalloc(&dbt, O * per * 8)) &&
+              HIP_OK(hipMalloc(&dbb, O * per * 8)) && HIP_OK(hipMalloc(&dbh, O * tg * n * 8)) &&
+              HIP_OK(hipMalloc(&dah, O * n * 8)) &&
+              HIP_OK(hipMemcpyAsync(dhint, hint, n * hint_cols * 8, hipMemcpyHostToDevice, st)) &&
+              offline_device(dhint, hint_cols, parts, w, v, n, tg, dbt, dbb, dbh, dah, st);
     ok = ok && HIP_OK(hipMemcpyAsync(bold_t, dbt, O * per * 8, hipMemcpyDeviceToHost, st)) &&
          HIP_OK(hipMemcpyAsync(bold_t_bar, dbb, O * per * 8, hipMemcpyDeviceToHost, st)) &&
          HIP_OK(hipMemcpyAsync(bold_t_hat, dbh, O * tg * n * 8, hipMemcpyDeviceToHost, st)) &&
          HIP_OK(hipMemcpyAsync(a_hat, dah, O * n * 8, hipMemcpyDeviceToHost, st)) && HIP_OK(hipStreamSynchronize(st));
-    for (void *p : {(void *)dhint, (void *)dA, (void *)dB, (void *)dr, (void *)drb, (void *)dr0, (void *)dbt,
-                    (void *)dbb, (void *)dbh, (void *)dah, (void *)dtab})
+    for (void *p : {(void *)dhint, (void *)dbt, (void *)dbb, (void *)dbh, (void *)dah})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
+// The whole setup from a database: hint, precomputation and packing matrix on the device, only
+// the database (when on the host), the query polynomials and masks in and U, the tail keys and
+// a_hat out.
+extern "C" int alto_scheme_setup(int8_t *ub, uint64_t *that, uint64_t *a_hat, const uint8_t *db_col, int on_device,
+                                 size_t ld, size_t rows, size_t cols, const uint64_t *a, size_t npolys,
+                                 const uint64_t *w, const uint64_t *v, size_t n, size_t tg) {
+    using namespace offline;
+    if (!ub || !that || !a_hat || !db_col || !a || !w || !v || ld < rows) {
+        set_err("alto_scheme_setup: null argument or ld < rows");
+        return -1;
+    }
+    if (!hint_shape_ok("alto_scheme_setup", rows, cols, n, npolys) || cols % n ||
+        !shape_ok("alto_scheme_setup", cols / n, n, tg, cols))
+        return -1;
+    const size_t K = tg * n, O = cols / n, num_iter = n / 2 - 1, per = num_iter * tg * n;
+    if (K % 64 || cols % 32 || K * 8 > kMaxLds) {
+        set_err("alto_scheme_setup: unsupported packing shape (tg n %% 64 == 0, columns %% 32 == 0)");
+        return -1;
+    }
+    int dev = 0;
+    if (!HIP_OK(hipGetDevice(&dev))) return -1;
+    int logn = 0;
+    while (((size_t)1 << logn) < n) logn++;
+    int8_t *db = nullptr, *dub = nullptr;
+    uint8_t *stage = nullptr;
+    uint64_t *dhint = nullptr, *dbt = nullptr, *dbb = nullptr, *dbh = nullptr, *dah = nullptr;
+    uint32_t *dtab = nullptr, *dpow = nullptr;
+    hipStream_t st = nullptr;
+    std::vector<uint32_t> gp(num_iter);
+    for (size_t i = 0; i < num_iter; i++) gp[i] = (uint32_t)(i ? gp[i - 1] * 5u % (2 * n) : 1u);
+    const size_t stage_cols = std::max<size_t>(16, std::min<size_t>(cols, ((size_t)256 << 20) / rows / 16 * 16));
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && HIP_OK(hipMalloc(&db, rows * cols)) &&
+              HIP_OK(hipMalloc(&stage, stage_cols * rows));
+    ok = ok && tile_db_columns(db_col, on_device != 0, ld, rows, cols, db, stage, stage_cols, st);
+    if (stage) (void)hipFree(stage);
+    ok = ok && HIP_OK(hipMalloc(&dhint, n * cols * 8)) && hint_device(db, rows, cols, n, a, npolys, dhint, st);
+    if (db) (void)hipFree(db);
+    ok = ok && HIP_OK(hipMalloc(&dbt, O * per * 8)) && HIP_OK(hipMalloc(&dbb, O * per * 8)) &&
+         HIP_OK(hipMalloc(&dbh, O * K * 8)) && HIP_OK(hipMalloc(&dah, O * n * 8)) &&
+         offline_device(dhint, cols, O, w, v, n, tg, dbt, dbb, dbh, dah, st);
+    if (dhint) (void)hipFree(dhint);
+    ok = ok && HIP_OK(hipMalloc(&dtab, n * n * 4)) && HIP_OK(hipMalloc(&dpow, num_iter * 4)) &&
+         HIP_OK(hipMalloc(&dub, 4 * K * cols)) &&
+         HIP_OK(hipMemcpyAsync(dpow, gp.data(), num_iter * 4, hipMemcpyHostToDevice, st));
+    if (ok) {
+        hipLaunchKernelGGL(tables_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, dtab, (int)n, logn);
+        ok = HIP_OK(hipGetLastError()) &&
+             pack_build_device(dbt, dbb, per, dtab, dpow, num_iter, tg, n, cols, kQ, dub, st) &&
+             HIP_OK(hipMemcpyAsync(ub, dub, 4 * K * cols, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipMemcpyAsync(that, dbh, O * K * 8, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipMemcpyAsync(a_hat, dah, O * n * 8, hipMemcpyDeviceToHost, st)) &&
+             HIP_OK(hipStreamSynchronize(st));
+    }
+    for (void *p : {(void *)dbt, (void *)dbb, (void *)dbh, (void *)dah, (void *)dtab, (void *)dpow, (void *)dub})
         if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
     return ok ? 0 : -1;

@@ -81,6 +81,28 @@ def device_offline(params: cl.Params, hint: np.ndarray) -> list[tuple]:
     return [(bt[o], bb[o], bh[o], ah[o]) for o in range(O)]
 
 
+def device_setup(params: cl.Params, db_col, ld: int | None = None, on_device: bool = False) -> workload.EngineInputs:
+    """The whole setup on the device in one call (alto_scheme_setup: hint, packing
+    precomputation and packing matrix without the intermediate results leaving the device):
+    the engine's inputs.  db_col a host array, or with on_device the address of the database in
+    device memory (then the returned inputs carry no database)."""
+    n, tg, O = params.n, params.tg, params.outputs
+    ld = params.rows if ld is None else ld
+    a = np.ascontiguousarray(cl.hint_query_polys(params), dtype=np.uint64)
+    w, v = (np.ascontiguousarray(x, dtype=np.uint64) for x in cl.masks(params))
+    ub = np.zeros(4 * tg * n * params.cols, np.int8)
+    that = np.zeros((O, tg * n), np.uint64)
+    a_hat = np.zeros((O, n), np.uint64)
+    lib = _lib.load()
+    rc = lib.alto_scheme_setup(ub.ctypes.data, that.ctypes.data, a_hat.ctypes.data,
+                               db_col if on_device else db_col.ctypes.data, 1 if on_device else 0, ld, params.rows,
+                               params.cols, a.ctypes.data, a.shape[0], w.ctypes.data, v.ctypes.data, n, tg)
+    if rc != 0:
+        raise RuntimeError(f"alto_scheme_setup failed ({_lib.last_error()})")
+    parts = [(None, None, that[o], a_hat[o]) for o in range(O)]
+    return server_inputs(params, None if on_device else db_col, parts, ub=ub)
+
+
 def server_inputs(params: cl.Params, db_col: np.ndarray, parts: list[tuple], ub: np.ndarray | None = None,
                   device: bool = True, threads: int = 16) -> workload.EngineInputs:
     """The engine's inputs from the per-part precomputation: U built from bold_t / bold_t_bar
@@ -118,6 +140,8 @@ def main(argv=None) -> int:
     ap.add_argument("--records", type=int, default=32768)
     ap.add_argument("--record-bytes", type=int, default=32768)
     ap.add_argument("--queries", type=int, default=3)
+    ap.add_argument("--staged", action="store_true",
+                    help="the three steps as separate calls, each result through host memory")
     args = ap.parse_args(argv)
     p = cl.Params(args.records, args.record_bytes)
     torch.cuda.set_device(0)
@@ -125,15 +149,19 @@ def main(argv=None) -> int:
     d_db = synthetic.uniform_bytes_device(0, p.rows * p.cols)
     torch.cuda.synchronize()
     t0 = time.time()
-    hint = device_hint(p, d_db.data_ptr(), on_device=True)
-    t["hint_s"] = time.time() - t0
-    t0 = time.time()
-    parts = device_offline(p, hint)
-    t["packing_precomputation_s"] = time.time() - t0
-    t0 = time.time()
-    inp = server_inputs(p, None, parts, device=True)
-    t["packing_matrix_s"] = time.time() - t0
-    del parts, hint
+    if args.staged:
+        hint = device_hint(p, d_db.data_ptr(), on_device=True)
+        t["hint_s"] = time.time() - t0
+        t0 = time.time()
+        parts = device_offline(p, hint)
+        t["packing_precomputation_s"] = time.time() - t0
+        t0 = time.time()
+        inp = server_inputs(p, None, parts, device=True)
+        t["packing_matrix_s"] = time.time() - t0
+        del parts, hint
+    else:
+        inp = device_setup(p, d_db.data_ptr(), on_device=True)
+        t["hint_precomputation_packing_matrix_s"] = time.time() - t0
     t0 = time.time()
     eng = Engine.from_device_db(d_db.data_ptr(), inp, threads=16)
     t["engine_s"] = time.time() - t0

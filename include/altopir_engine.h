@@ -100,6 +100,19 @@ int alto_inspiring_offline(uint64_t *bold_t, uint64_t *bold_t_bar, uint64_t *bol
                            const uint64_t *hint, size_t hint_cols, size_t parts, const uint64_t *w,
                            const uint64_t *v, size_t n, size_t tg);
 
+/* No counterpart in the reference's C ABI: the server's whole offline setup in one call
+ * (src/service.rs:129-157, Prepared::new: perform_offline_precomputation_simplepir, then
+ * build_packing_matrix), on the device: the hint as alto_hint0 computes it (db_col, on_device,
+ * ld, rows, cols, a, npolys as there), the packing precomputation as alto_inspiring_offline
+ * (w, v, n, tg as there, parts = cols / n) and the packing matrix from it as alto_pack_build
+ * (the automorphism tables of the same NTT, generator 5, num_iter = n/2 - 1, Q), without the
+ * hint or the aggregates leaving the device.  Outputs (host): ub (4 tg n cols bytes, the layout
+ * of sw_pack_build), that (tg n per part: bold_t_hat, the engine's tail keys), a_hat (n per
+ * part).  Returns 0, or -1 (alto_last_error says why). */
+int alto_scheme_setup(int8_t *ub, uint64_t *that, uint64_t *a_hat, const uint8_t *db_col, int on_device, size_t ld,
+                      size_t rows, size_t cols, const uint64_t *a, size_t npolys, const uint64_t *w,
+                      const uint64_t *v, size_t n, size_t tg);
+
 /* replaces alto_engine_new, src/amx_kernels.cpp:546-560 (src/engine.rs:12-14).
  * Returns nullptr for the configurations the reference rejects (Q != 4294955009,
  * n == 0 or n > 4096 or n % 16, cols % n, rows % 64, cols % 32, threads <= 0,

@@ -47,7 +47,7 @@ def test_device_setup_and_engine_answers_decrypt_to_the_records(hip_device, rows
     p = cl.Params(rows, cols, 2048, 4)
     db = synthetic.uniform_bytes(7, rows * cols).reshape(cols, rows)
     db_col = np.ascontiguousarray(db).reshape(-1)
-    inp = _device_server(p, db_col)
+    inp = offline.device_setup(p, db_col) if rows >= 32768 else _device_server(p, db_col)
     eng = Engine.from_inputs(inp)
     clients, reqs = [], []
     for k, i in enumerate(idx):
@@ -76,3 +76,24 @@ def test_device_offline_rejects_unsupported_shapes(hip_device):
                                     3, z.ctypes.data, z.ctypes.data, 32, 4)
     assert rc == -1 and "unsupported shape" in _lib.last_error()
     assert offline.device_offline(p, hint) is not None
+
+
+@pytest.mark.parametrize("n,rows,cols,on_device", [(256, 1024, 512, False), (2048, 4096, 4096, True)])
+def test_one_call_setup_equals_the_staged_setup(hip_device, n, rows, cols, on_device):
+    # alto_scheme_setup keeps the hint and the aggregates on the device; its packing matrix, tail
+    # keys and prefixes equal those of the three separate calls
+    import torch
+
+    p = cl.Params(rows, cols, n, 4)
+    db_col = synthetic.uniform_bytes(n + 3, rows * cols)
+    staged = _device_server(p, db_col)
+    if on_device:
+        d = torch.from_numpy(db_col.copy()).to("cuda")
+        one = offline.device_setup(p, d.data_ptr(), on_device=True)
+    else:
+        one = offline.device_setup(p, db_col)
+    assert np.array_equal(one.ub, staged.ub)
+    for o in range(p.outputs):
+        assert np.array_equal(one.that[o], staged.that[o]), o
+        assert np.array_equal(one.prefixes[o], staged.prefixes[o]), o
+    assert np.array_equal(one.inv_table, staged.inv_table)
