@@ -180,25 +180,25 @@ void launch_dma2(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
                            (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
-template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool WIDE = false>
 void prepare_dmab() {
-    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false>::kLds;
-    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>,
+    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false, WIDE>::kLds;
+    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true, WIDE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false>,
+    (void)hipFuncSetAttribute((const void *)dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false, WIDE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool WIDE = false>
 void launch_dmab(GemmArgs g, const int8_t *a, const int8_t *b, hipStream_t st) {
-    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false>::kLds;
+    constexpr size_t lds = DmabShape<TM, TN, WM, WN, D, UNR, PACK, false, WIDE>::kLds;
     const int grid = plan_ring<TM, TN, WM, WN>(g, PACK);
     if (g.gz > 1)
-        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true>), dim3(grid), dim3(64 * WM * WN), lds,
-                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, true, WIDE>), dim3(grid),
+                           dim3(64 * WM * WN), lds, st, (const i32x4 *)a, (const i32x4 *)b, g);
     else
-        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false>), dim3(grid), dim3(64 * WM * WN), lds,
-                           st, (const i32x4 *)a, (const i32x4 *)b, g);
+        hipLaunchKernelGGL((dmab_kernel<TM, TN, WM, WN, D, UNR, PACK, false, WIDE>), dim3(grid),
+                           dim3(64 * WM * WN), lds, st, (const i32x4 *)a, (const i32x4 *)b, g);
 }
 
 template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK>
@@ -307,6 +307,10 @@ const Variant kVariants[] = {
      prepare_dmab<8, 4, 2, 4, 2, 8, false>, prepare_dmab<8, 4, 2, 4, 2, 8, true>},
     {"dmab8_m8_n4_u16", launch_dmab<8, 4, 2, 4, 2, 16, false>, launch_dmab<8, 4, 2, 4, 2, 16, true>,
      prepare_dmab<8, 4, 2, 4, 2, 16, false>, prepare_dmab<8, 4, 2, 4, 2, 16, true>},
+    // the same on 32 x 32 x 32 MFMAs (tile pairs): the same bytes, 25% slower at 1 and 8 GiB
+    // (batches of 40-2048)
+    {"dmaw8_m8_n4_u16", launch_dmab<8, 4, 2, 4, 2, 16, false, true>, launch_dmab<8, 4, 2, 4, 2, 16, true, true>,
+     prepare_dmab<8, 4, 2, 4, 2, 16, false, true>, prepare_dmab<8, 4, 2, 4, 2, 16, true, true>},
     // the same with the A fragments read from LDS just before their MFMAs
     {"dmac8_m8_n4_d2u16", launch_dmac<8, 4, 2, 4, 2, 16, false>, launch_dmac<8, 4, 2, 4, 2, 16, true>,
      prepare_dmac<8, 4, 2, 4, 2, 16, false>, prepare_dmac<8, 4, 2, 4, 2, 16, true>},

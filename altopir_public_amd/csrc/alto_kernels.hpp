@@ -40,6 +40,7 @@ constexpr int kBlockBytes = 1024;
 constexpr int kFoldSteps = 1024;
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 // ------------------------------------------------------------------ mod-Q helpers --
 
@@ -566,10 +567,9 @@ __device__ __forceinline__ uint32_t recombine_u32(i32x4 a) {
     return (uint32_t)a.x + ((uint32_t)a.y << 8) + ((uint32_t)a.z << 16) + ((uint32_t)a.w << 24);
 }
 
+// The four limb sums of query c against column col.
 template <bool ATOMIC>
-__device__ __forceinline__ void scan_store(const GemmArgs &g, int rt, int ct, i32x4 a) {
-    const int lane = threadIdx.x & 63;
-    const int c = 4 * rt + (lane >> 4), col = 16 * ct + (lane & 15);
+__device__ __forceinline__ void scan_store_at(const GemmArgs &g, int c, int col, i32x4 a) {
     if (g.out64) {  // sum_i 256^i P_i exactly (|P_i| <= 2^30 per split), reduced mod Q
         const int64_t t = (int64_t)a.x + ((int64_t)a.y << 8) + ((int64_t)a.z << 16) + ((int64_t)a.w << 24);
         unsigned long long *d = g.out64 + (size_t)c * g.C + col;
@@ -581,25 +581,32 @@ __device__ __forceinline__ void scan_store(const GemmArgs &g, int rt, int ct, i3
     if (ATOMIC) atomicAdd(dst, recombine_u32(a));
     else *dst = recombine_u32(a);
 }
+// A 16 x 16 output tile (A tile rt, B tile ct): lane l holds the limb rows 4 (l >> 4) .. + 3
+// of column l & 15.
+template <bool ATOMIC>
+__device__ __forceinline__ void scan_store(const GemmArgs &g, int rt, int ct, i32x4 a) {
+    const int lane = threadIdx.x & 63;
+    scan_store_at<ATOMIC>(g, 4 * rt + (lane >> 4), 16 * ct + (lane & 15), a);
+}
 
-// beta = sum_{ij} 2^{8(i+j)} G_ij + ucorr[col] + 128E * Ysum[c] - (128E)^2 KP  (mod Q);
+// beta = sum_{ij} 2^{8(i+j)} G_ij + ucorr[col] + 128E * Ysum[c] - (128E)^2 KP  (mod Q), G_ij the
+// product of limb row i of query c with byte plane j of the packing matrix.
+// pack_plane: sum_i 2^{8(i+j)} G_ij over the four limb rows i (a = G_0j .. G_3j), with 2^{8s}
+// for s = 4, 5, 6 replaced by its residue 12287 * 2^{8(s-4)} mod Q; |G_ij| <= 2^30 (tg * n <=
+// 65536), so one plane's sum stays below 2^60 and the four planes' below 2^62.
+__device__ __forceinline__ int64_t pack_plane(i32x4 a, int j) {
+    const int64_t w[7] = {1, 1ll << 8, 1ll << 16, 1ll << 24, 12287, 12287ll << 8, 12287ll << 16};
+    return a.x * w[j] + a.y * w[j + 1] + a.z * w[j + 2] + a.w * w[j + 3];
+}
+__device__ __forceinline__ void pack_store_at(const GemmArgs &g, int c, int col, int64_t t) {
+    const uint32_t yc = addmod(mulmod(red64(g.ysum[c]), g.k128e), g.ycorr0);
+    g.out[(size_t)c * g.C + col] = addmod(addmod(red_i64(t), g.ucorr[col]), yc);
+}
 // a[j] holds G_ij of plane j for the four limb rows i of this lane's query.
 __device__ __forceinline__ void pack_store(const GemmArgs &g, int rt, int ct, const i32x4 *a) {
     const int lane = threadIdx.x & 63;
-    const int c = 4 * rt + (lane >> 4), col = 16 * ct + (lane & 15);
-    int64_t V[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        V[j + 0] += a[j].x;
-        V[j + 1] += a[j].y;
-        V[j + 2] += a[j].z;
-        V[j + 3] += a[j].w;
-    }
-    // 2^{8s} mod Q for s = 4, 5, 6 (s <= 3 are exact powers)
-    const int64_t w4 = 12287, w5 = 12287ll << 8, w6 = 12287ll << 16;
-    const int64_t t = V[0] + (V[1] << 8) + (V[2] << 16) + (V[3] << 24) + V[4] * w4 + V[5] * w5 + V[6] * w6;
-    const uint32_t yc = addmod(mulmod(red64(g.ysum[c]), g.k128e), g.ycorr0);
-    g.out[(size_t)c * g.C + col] = addmod(addmod(red_i64(t), g.ucorr[col]), yc);
+    const int64_t t = pack_plane(a[0], 0) + pack_plane(a[1], 1) + pack_plane(a[2], 2) + pack_plane(a[3], 3);
+    pack_store_at(g, 4 * rt + (lane >> 4), 16 * ct + (lane & 15), t);
 }
 
 // Streaming contraction for few queries (HBM-bound): operands straight from memory into
@@ -1039,13 +1046,24 @@ __device__ __align__(16) i32x4 kZeroBlock[64];
 // the top of each group, where it waits for every load): steps past the end of the range
 // reload the last A block into slots no remaining step reads and take their B blocks from
 // kZeroBlock, so they add nothing; all loads are waited for before the workgroup ends.
-template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+//
+// WIDE: the same data on 32 x 32 x 32 MFMAs, each on a pair of A tiles and a pair of B tiles
+// (half the instructions, each twice the work; the same operand bytes per wave and step).
+// A 1 KiB block holds a 16 x 64 operand in four k-groups of 16 bytes (lane l: row l & 15,
+// k-group l >> 4).  The 32 x 32 x 32 operand of half h of a step takes, in lane l, row (or
+// column) l & 15 of tile (l >> 4) & 1 of the pair and k-group (l >> 5) + 2h: both operands use
+// the same k-group for the same lane half, so the two halves sum over all 64 k of the step
+// whatever order the hardware gives the bytes of a k-group.  Lane l of the i32x16 result holds
+// column l & 31 and, in registers 4G .. 4G + 3, the limb rows 8G + 4 (l >> 5) .. + 3: query
+// 2 (G & 1) + (l >> 5) of A tile G >> 1 of the pair.
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC, bool WIDE = false>
 __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const i32x4 *__restrict__ b_src,
                                           const GemmArgs &g, int L) {
     constexpr int NW = WM * WN;
     static_assert(NW == 8, "two waves per SIMD");
     constexpr int BM = WM * TM;
     static_assert(!PACK || TN % 4 == 0, "packing tiles come in groups of four planes");
+    static_assert(!WIDE || (TM % 2 == 0 && TN % 2 == 0), "32 x 32 tiles pair the 16 x 16 tiles");
     static_assert(D >= 2 && D <= 3, "prefetch distance");
     static_assert(UNR % D == 0, "whole register rings per loop iteration");
     constexpr int S = D + 2;
@@ -1063,12 +1081,20 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
     const int MT = batch_tiles(g);
     if (mt0 >= MT) return;  // a row block past the batch (launch sized for its bucket)
     const int kb = (int)((long long)g.KC * tm.z / g.gz), ke = (int)((long long)g.KC * (tm.z + 1) / g.gz);
+    // fragment m (B fragment n) of a step: WIDE, half m & 1 of the tile pair m >> 1 (tile of
+    // the pair: sel), else tile m; frag_at gives its 16-byte unit within the wave's blocks
+    const int sel = WIDE ? (lane >> 4) & 1 : 0;
+    const int e0 = WIDE ? (lane & 15) + 16 * (lane >> 5) : lane;
+    auto frag_at = [&](int m) { return WIDE ? (2 * (m >> 1) + sel) * 64 + e0 + 32 * (m & 1) : m * 64 + lane; };
     const i32x4 *asrc[PA];
 #pragma unroll
     for (int p = 0; p < PA; p++) asrc[p] = a_src + (size_t)min(mt0 + wave + NW * p, MT - 1) * g.KC * 64 + lane;
     const i32x4 *bsrc[TN];
 #pragma unroll
-    for (int n = 0; n < TN; n++) bsrc[n] = b_block<PACK>(b_src, min(bt0 + n, g.NT - 1), g.KC) + lane;
+    for (int n = 0; n < TN; n++) {
+        const int t = WIDE ? bt0 + 2 * (n >> 1) + sel : bt0 + n;
+        bsrc[n] = b_block<PACK>(b_src, min(t, g.NT - 1), g.KC) + (WIDE ? e0 + 32 * (n & 1) : lane);
+    }
     const i32x4 *zero = kZeroBlock + lane;
     auto issue_a = [&](int t) {  // A(t) into its slot (t past the range: the last block)
         i32x4 *dst = ring + (size_t)(((t - kb) % S) * BMP) * 64;
@@ -1083,11 +1109,13 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
 #pragma unroll
         for (int n = 0; n < TN; n++) b[n] = *(live ? bsrc[n] + (size_t)t * BS : zero);
     };
-    i32x4 acc[TM][TN];
+    constexpr int AM = WIDE ? TM / 2 : TM, AN = WIDE ? TN / 2 : TN;
+    typedef int AccT __attribute__((ext_vector_type(WIDE ? 16 : 4)));
+    AccT acc[AM][AN];
 #pragma unroll
-    for (int m = 0; m < TM; m++)
+    for (int m = 0; m < AM; m++)
 #pragma unroll
-        for (int n = 0; n < TN; n++) acc[m][n] = i32x4{0, 0, 0, 0};
+        for (int n = 0; n < AN; n++) acc[m][n] = (AccT)0;
     if (ke > kb) {  // an empty record range leaves the accumulators zero
         i32x4 a[TM];
         i32x4 breg[D][TN];
@@ -1107,9 +1135,9 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
         wait_vmcnt<0>();
         stage_barrier();
         {
-            const i32x4 *cur = ring + (size_t)(wm * TM) * 64 + lane;
+            const i32x4 *cur = ring + (size_t)(wm * TM) * 64;
 #pragma unroll
-            for (int m = 0; m < TM; m++) a[m] = cur[m * 64];
+            for (int m = 0; m < TM; m++) a[m] = cur[frag_at(m)];
         }
         const int kend = kb + (ke - kb + UNR - 1) / UNR * UNR;
         for (int k0 = kb; k0 < kend; k0 += UNR) {
@@ -1119,17 +1147,31 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
                 const int k = k0 + jj;
                 wait_vmcnt<(D - 1) * PAIR>();  // P(k): A(k + 1) and B(k)
                 stage_barrier();               // A(k + 1) visible; slot of step k - 1 free
-                const i32x4 *nxt = ring + (size_t)(((k + 1 - kb) % S) * BMP + wm * TM) * 64 + lane;
+                const i32x4 *nxt = ring + (size_t)(((k + 1 - kb) % S) * BMP + wm * TM) * 64;
                 // kept in this order (scheduling barriers): hoisting a fragment read or the B
                 // loads above the MFMAs that free their registers would double the live
                 // operands
+                if constexpr (WIDE) {
 #pragma unroll
-                for (int m = 0; m < TM; m++) {
+                    for (int h = 0; h < 2; h++)
 #pragma unroll
-                    for (int n = 0; n < TN; n++)
-                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], breg[j][n], acc[m][n], 0, 0, 0);
-                    a[m] = nxt[m * 64];  // step k + 1's fragment (past the range: unused)
-                    __builtin_amdgcn_sched_barrier(0);
+                        for (int mp = 0; mp < AM; mp++) {
+#pragma unroll
+                            for (int np = 0; np < AN; np++)
+                                acc[mp][np] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * mp + h], breg[j][2 * np + h],
+                                                                                    acc[mp][np], 0, 0, 0);
+                            a[2 * mp + h] = nxt[frag_at(2 * mp + h)];
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < TM; m++) {
+#pragma unroll
+                        for (int n = 0; n < TN; n++)
+                            acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[m], breg[j][n], acc[m][n], 0, 0, 0);
+                        a[m] = nxt[frag_at(m)];  // step k + 1's fragment (past the range: unused)
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
                 issue_a(k + 1 + D);  // into the slot of step k - 1
                 load_b(k + D, breg[j]);
@@ -1138,29 +1180,64 @@ __device__ __forceinline__ void dmab_tile(const i32x4 *__restrict__ a_src, const
         }
         wait_vmcnt<0>();  // loads past the range land before the workgroup's LDS is released
     }
+    if constexpr (WIDE) {
+        const int h = lane >> 5, col = lane & 15;
 #pragma unroll
-    for (int m = 0; m < TM; m++) {
-        const int rt = mt0 + wm * TM + m;
-        if (rt >= MT) continue;
-        if (PACK) {
+        for (int mp = 0; mp < AM; mp++)
 #pragma unroll
-            for (int n = 0; n < TN; n += 4) {
-                const int t = bt0 + n;
-                if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+            for (int G = 0; G < 4; G++) {
+                const int rt = mt0 + wm * TM + 2 * mp + (G >> 1);
+                const int c = 4 * rt + 2 * (G & 1) + h;
+                if (PACK) {
+                    // B pair np holds planes 2 (np & 1) + sel of column tile (bt0 + 2 np) / 4: a
+                    // lane has two of the four planes, its partner (lane ^ 16) the other two
+#pragma unroll
+                    for (int np = 0; np < AN; np += 2) {
+                        const i32x4 p0 = {acc[mp][np][4 * G], acc[mp][np][4 * G + 1], acc[mp][np][4 * G + 2],
+                                          acc[mp][np][4 * G + 3]};
+                        const i32x4 p1 = {acc[mp][np + 1][4 * G], acc[mp][np + 1][4 * G + 1],
+                                          acc[mp][np + 1][4 * G + 2], acc[mp][np + 1][4 * G + 3]};
+                        int64_t t = pack_plane(p0, sel) + pack_plane(p1, 2 + sel);
+                        t += __shfl_xor(t, 16);
+                        const int tt = bt0 + 2 * np;
+                        if (sel == 0 && rt < MT && tt < g.NT) pack_store_at(g, c, 16 * (tt >> 2) + col, t);
+                    }
+                } else {
+                    if (rt >= MT) continue;
+#pragma unroll
+                    for (int np = 0; np < AN; np++) {
+                        const int t = bt0 + 2 * np + sel;
+                        const i32x4 v = {acc[mp][np][4 * G], acc[mp][np][4 * G + 1], acc[mp][np][4 * G + 2],
+                                         acc[mp][np][4 * G + 3]};
+                        if (t < g.NT) scan_store_at<ATOMIC>(g, c, 16 * t + col, v);
+                    }
+                }
             }
-        } else {
+    } else {
 #pragma unroll
-            for (int n = 0; n < TN; n++) {
-                const int t = bt0 + n;
-                if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+        for (int m = 0; m < TM; m++) {
+            const int rt = mt0 + wm * TM + m;
+            if (rt >= MT) continue;
+            if (PACK) {
+#pragma unroll
+                for (int n = 0; n < TN; n += 4) {
+                    const int t = bt0 + n;
+                    if (t < g.NT) pack_store(g, rt, t >> 2, &acc[m][n]);
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < TN; n++) {
+                    const int t = bt0 + n;
+                    if (t < g.NT) scan_store<ATOMIC>(g, rt, t, acc[m][n]);
+                }
             }
         }
     }
 }
-template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC, bool WIDE = false>
 __global__ __launch_bounds__(64 * WM * WN, 1) void dmab_kernel(const i32x4 *__restrict__ a_src,
                                                                const i32x4 *__restrict__ b_src, GemmArgs g) {
-    dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a_src, b_src, g, blockIdx.x);
+    dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC, WIDE>(a_src, b_src, g, blockIdx.x);
 }
 
 // The register-streaming contraction with the A fragments read from LDS just before their
@@ -1298,12 +1375,12 @@ struct DmaShape {
     }
 };
 
-template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC>
+template <int TM, int TN, int WM, int WN, int D, int UNR, bool PACK, bool ATOMIC, bool WIDE = false>
 struct DmabShape {
     static constexpr int kThreads = 64 * WM * WN;
     static constexpr size_t kLds = (size_t)(D + 2) * ((WM * TM + 7) / 8 * 8) * kBlockBytes;
     static __device__ __forceinline__ void tile(const i32x4 *a, const i32x4 *b, const GemmArgs &g, int L) {
-        dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC>(a, b, g, L);
+        dmab_tile<TM, TN, WM, WN, D, UNR, PACK, ATOMIC, WIDE>(a, b, g, L);
     }
 };
 
