@@ -47,16 +47,26 @@ typedef int i32x16 __attribute__((ext_vector_type(16)));
 // x < 2^32 + 2^28 -> x mod Q
 __device__ __forceinline__ uint32_t red_once(uint64_t x) { return (uint32_t)(x >= kQ ? x - kQ : x); }
 
-// any u64 -> mod Q, using 2^32 == 12287 (mod Q) twice
+// s + carry * 2^32 (< 2Q) -> mod Q in 32-bit arithmetic: subtracting Q is adding 12287 mod 2^32
+__device__ __forceinline__ uint32_t red_carry(uint32_t s, bool carry) { return (carry || s >= kQ) ? s + (uint32_t)kFold : s; }
+
+// any u64 -> mod Q, using 2^32 == 12287 (mod Q) twice: p = H 2^32 + L == H 12287 + L = t (one
+// 32 x 32 + 32 -> 64 multiply-add), t = T1 2^32 + T0 == T1 12287 + T0 with T1 <= 2^14 (a 24-bit
+// multiply), < 2^32 + 2^28 < 2Q
 __device__ __forceinline__ uint32_t red64(uint64_t p) {
-    uint64_t t = (p >> 32) * kFold + (p & 0xffffffffull);
-    t = (t >> 32) * kFold + (t & 0xffffffffull);
-    return red_once(t);
+    const uint64_t t = (p >> 32) * kFold + (p & 0xffffffffull);
+    uint32_t s;
+    const bool carry = __builtin_add_overflow((uint32_t)t, __umul24((uint32_t)(t >> 32), (uint32_t)kFold), &s);
+    return red_carry(s, carry);
 }
 
 __device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b) { return red64((uint64_t)a * b); }
 
-__device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b) { return red_once((uint64_t)a + b); }
+__device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b) {
+    uint32_t s;
+    const bool carry = __builtin_add_overflow(a, b, &s);
+    return red_carry(s, carry);
+}
 
 // signed |t| < 2^62 -> t mod Q in [0, Q)
 __device__ __forceinline__ uint32_t red_i64(int64_t t) {
@@ -1431,41 +1441,41 @@ struct TailArgs {
     const int *dims;        // {B, MT} in device memory (launch sized for the batch's bucket), or null
 };
 
-// trunc(x / Q) for |x| < 2^53, exactly: a double-precision estimate (off by at most one
-// near an integer quotient) corrected with the exact integer remainder.
-__device__ __forceinline__ int64_t div_q_trunc(int64_t x) {
-    const int64_t Q = (int64_t)kQ;
-    int64_t t = (int64_t)((double)x * (1.0 / (double)kQ));
-    const int64_t r = x - t * Q;
-    if (x >= 0) {
-        if (r < 0) t--;
-        else if (r >= Q) t++;
-    } else {
-        if (r > 0) t++;
-        else if (r <= -Q) t--;
-    }
-    return t;
+// floor(x / Q) for 0 <= x < 2^53, exactly: x = T0 2^32 + L = T0 Q + (T0 12287 + L), the
+// remainder form r = T1 2^32 + L1 (T1 <= 8) = T1 Q + (T1 12287 + L1), and the last term is
+// below 2Q.
+__device__ __forceinline__ uint64_t div_q_floor(uint64_t x) {
+    const uint64_t t0 = x >> 32;
+    const uint64_t r = t0 * kFold + (x & 0xffffffffull);
+    const uint32_t t1 = (uint32_t)(r >> 32);
+    uint32_t s;
+    const bool carry = __builtin_add_overflow((uint32_t)r, __umul24(t1, (uint32_t)kFold), &s);
+    return t0 + t1 + ((carry || s >= kQ) ? 1 : 0);
 }
 
 // Row-1 rescale Q -> q1 (src/amx_kernels.cpp:522-529): centre, round half away from zero,
-// wrap into [0, q1).  Exact integer form of the reference's double-precision division.
+// wrap into [0, q1).  Exact integer form of the reference's double-precision division: with
+// the centred value c and h = floor(Q / 2), trunc((c q1 + sign(c) h) / Q) = sign(c) floor((|c|
+// q1 + h) / Q), and |c| q1 + h < 2^53 for q1 < 2^21.
 __device__ __forceinline__ uint32_t rescale_q1(uint32_t v, uint32_t q1) {
-    const int64_t half = (int64_t)(kQ / 2);
-    const bool neg = (int64_t)v >= half;
-    const int64_t c = neg ? (int64_t)v - (int64_t)kQ : (int64_t)v;
-    const int64_t x = c * (int64_t)q1 + (neg ? -half : half);
-    int64_t t = div_q_trunc(x);
-    if (t < 0) t += q1;
-    if (t >= (int64_t)q1) t -= q1;
-    return (uint32_t)t;
+    const uint32_t half = (uint32_t)(kQ / 2);
+    const bool neg = v >= half;
+    const uint32_t a = neg ? (uint32_t)kQ - v : v;
+    const uint32_t u = (uint32_t)div_q_floor((uint64_t)a * q1 + half);
+    uint32_t t = neg ? (u ? q1 - u : 0) : u;
+    if (t >= q1) t -= q1;
+    return t;
 }
 
+// Gentleman-Sande butterfly with the halving: x <- (x + y) / 2, y <- (x - y) w (mod Q), in 32-bit
+// arithmetic (x + y < 2Q as a carry and 32 bits; (s + Q) / 2 = s / 2 + (Q + 1) / 2 for odd s).
 __device__ __forceinline__ void gs_butterfly(uint32_t &x, uint32_t &y, uint32_t w) {
-    uint64_t s = (uint64_t)x + y;
-    if (s >= kQ) s -= kQ;
-    s = (s & 1) ? (s + kQ) >> 1 : s >> 1;  // (x + y) / 2 mod Q
-    const uint32_t d = red_once((uint64_t)x + kQ - y);
-    x = (uint32_t)s;
+    uint32_t s;
+    const bool carry = __builtin_add_overflow(x, y, &s);
+    s = red_carry(s, carry);
+    s = (s >> 1) + ((s & 1) ? (uint32_t)(kQ / 2 + 1) : 0u);
+    const uint32_t d = x >= y ? x - y : x - y - (uint32_t)kFold;  // x - y + Q mod 2^32
+    x = s;
     y = mulmod(d, w);
 }
 
@@ -1522,11 +1532,21 @@ __device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool 
             const int base = ((g >> ls) << (ls + 3)) | (g & (sp - 1));
             uint32_t v[8];
             if (ph == 0) {
-                // pointwise: a = beta + sum_k z * that (mod Q), elements 8g .. 8g + 7
-                uint64_t s8[8];
+                // pointwise: a = beta + sum_k z * that (mod Q), elements 8g .. 8g + 7; the
+                // products' high and low words summed apart (each sum < 2^32 (tg + 1)) and
+                // reduced once: sum hi 2^32 + sum lo == sum hi 12287 + sum lo, below 2^59 for
+                // tg * n <= 65536
+                uint64_t s8[8], h8[8];
                 const uint4 b0 = ld4(bc + base), b1 = ld4(bc + base + 4);
                 s8[0] = b0.x; s8[1] = b0.y; s8[2] = b0.z; s8[3] = b0.w;
                 s8[4] = b1.x; s8[5] = b1.y; s8[6] = b1.z; s8[7] = b1.w;
+#pragma unroll
+                for (int m = 0; m < 8; m++) h8[m] = 0;
+                auto mac = [&](int m, uint32_t z, uint32_t t) {
+                    const uint64_t pr = (uint64_t)z * t;
+                    s8[m] += (uint32_t)pr;
+                    h8[m] += pr >> 32;
+                };
                 // keys KB at a time: all their loads are issued before the first product
                 for (int k0 = 0; k0 < p.tg; k0 += KB) {
                     uint4 z0[KB], z1[KB], t0[KB], t1[KB];
@@ -1541,14 +1561,14 @@ __device__ __forceinline__ void tail_task(const TailArgs &p, int o, int c, bool 
 #pragma unroll
                     for (int j = 0; j < KB; j++) {
                         if (k0 + j >= p.tg) break;
-                        s8[0] += mulmod(z0[j].x, t0[j].x); s8[1] += mulmod(z0[j].y, t0[j].y);
-                        s8[2] += mulmod(z0[j].z, t0[j].z); s8[3] += mulmod(z0[j].w, t0[j].w);
-                        s8[4] += mulmod(z1[j].x, t1[j].x); s8[5] += mulmod(z1[j].y, t1[j].y);
-                        s8[6] += mulmod(z1[j].z, t1[j].z); s8[7] += mulmod(z1[j].w, t1[j].w);
+                        mac(0, z0[j].x, t0[j].x); mac(1, z0[j].y, t0[j].y);
+                        mac(2, z0[j].z, t0[j].z); mac(3, z0[j].w, t0[j].w);
+                        mac(4, z1[j].x, t1[j].x); mac(5, z1[j].y, t1[j].y);
+                        mac(6, z1[j].z, t1[j].z); mac(7, z1[j].w, t1[j].w);
                     }
                 }
 #pragma unroll
-                for (int m = 0; m < 8; m++) v[m] = red64(s8[m]);
+                for (int m = 0; m < 8; m++) v[m] = red64(h8[m] * kFold + s8[m]);
             } else {
 #pragma unroll
                 for (int m = 0; m < 8; m++) v[m] = a[base + (m << ls)];
