@@ -1036,9 +1036,9 @@ int launch_batch(const Engine &E, const Workspace &w, FrameSet fs, size_t B, uin
     pa.acc = (uint4 *)acc;
     pa.acc4 = Bp * E.cols / 4;
     pa.sx = (unsigned)((E.KC * 4 + 255) / 256);
-    pa.sy = (unsigned)Bp;
+    pa.sy = (unsigned)(Bp / kPrepClients);
     pa.px = (unsigned)((E.KCp * 4 + 255) / 256);
-    pa.py = (unsigned)Bp;
+    pa.py = (unsigned)(Bp / kPrepClients);
     pa.nfill = (unsigned)std::min<size_t>(1024, (pa.acc4 + 1023) / 1024);
     pa.sync = w.sync;
     pa.nsync = (unsigned)sync_words(E);
@@ -1297,7 +1297,7 @@ int packed_product(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t r
         const size_t work = CT * KC * 64;
         hipLaunchKernelGGL(tile_db_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, (const uint8_t *)stage,
                            P, db, CT, KC, (size_t)0);
-        hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)Bp), dim3(256), 0,
+        hipLaunchKernelGGL((limb_prep_kernel<true>), dim3((unsigned)((KC * 4 + 255) / 256), (unsigned)(Bp / kPrepClients)), dim3(256), 0,
                            st, FrameSet{slots, S, nullptr}, (size_t)0, (const int *)status, (int)bc, KC, qa, qsum, ysum);
         GemmArgs g{};
         g.KC = (int)KC;

@@ -329,54 +329,92 @@ __device__ __forceinline__ T block_sum(T v, T *red) {
     return s;
 }
 
+// Clients whose limb rows one thread of the limb preparation writes: 2 makes each thread's
+// stores one whole 128-byte line of an A block.
+constexpr int kPrepClients = 2;
+
 // Splits u32 words into four byte-limb rows of an MFMA A operand (rows 4c+i, K = nwords).
 // SCAN: signed digits of q + 0x80808080 and sum(q) mod 2^32 into qsum.
 // PACK: bytes of y offset by -128 and the exact sum(y) into ysum.
-// grid = (ceil(KC * 4 / 256), Bp); clients c >= B or with status != 0 get zero rows.
-// Block (bx, by) of that grid; limb_prep_kernel runs the grid, prep_kernel runs both limb
-// preparations and the accumulator clear in one launch.
+// grid = (ceil(KC * 4 / 256), Bp / kPrepClients): block (bx, by) takes the clients
+// kPrepClients * by + q (q < kPrepClients); thread p of the x range takes k-step p >> 2, k-group
+// p & 3 of each of them, 16 words of each client, and writes the 16 * kPrepClients contiguous
+// 4-byte words those make in the A block (whole cache lines).  All the frame loads of a thread
+// are issued before its first store.  Clients c >= B or with status != 0 get zero rows.
+// limb_prep_kernel runs the grid, prep_kernel runs both limb preparations and the accumulator
+// clear in one launch.
 template <bool SCAN>
 __device__ __forceinline__ void limb_prep_block(FrameSet fs, size_t word_off, const int *__restrict__ status, int B,
                                                 size_t KC, int8_t *__restrict__ dst, uint32_t *__restrict__ qsum,
                                                 unsigned long long *__restrict__ ysum, unsigned bx, unsigned by) {
-    __shared__ unsigned long long red[8];
-    const int c = (int)by;
+    constexpr int CP = kPrepClients;
+    static_assert(4 % CP == 0, "clients of a thread lie in one A tile");
+    __shared__ unsigned long long red[16][CP];  // [wave][client of the block]
+    const int c0 = CP * (int)by;
     const size_t p = (size_t)bx * blockDim.x + threadIdx.x;
     const size_t kc = p >> 2;
     const int g = (int)(p & 3);
-    const bool ok = c < B && status[c] == 0;
-    unsigned long long s = 0;
+    unsigned long long s[CP];
+#pragma unroll
+    for (int q = 0; q < CP; q++) s[q] = 0;
     if (kc < KC) {
-        uint4 o[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-        if (ok) {
-            const uint4 *src = (const uint4 *)(fs.slot(c) + 16 + word_off * 4) + (kc * 64 + g * 16) / 4;
+        bool ok[CP];
+        uint4 v[CP][4];
+#pragma unroll
+        for (int q = 0; q < CP; q++) {
+            const int c = c0 + q;
+            ok[q] = c < B && status[c] == 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[q][e] = make_uint4(0, 0, 0, 0);
+            if (ok[q]) {
+                const uint4 *src = (const uint4 *)(fs.slot(c) + 16 + word_off * 4) + (kc * 64 + g * 16) / 4;
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[q][e] = src[e];
+            }
+        }
+        uint4 *blk = (uint4 *)(dst + ((size_t)(c0 >> 2) * KC + kc) * kBlockBytes) + 16 * g + 4 * (c0 & 3);
+#pragma unroll
+        for (int q = 0; q < CP; q++) {
             uint32_t w[16];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const uint4 v = src[e];
-                w[4 * e] = v.x; w[4 * e + 1] = v.y; w[4 * e + 2] = v.z; w[4 * e + 3] = v.w;
+                w[4 * e] = v[q][e].x; w[4 * e + 1] = v[q][e].y; w[4 * e + 2] = v[q][e].z; w[4 * e + 3] = v[q][e].w;
+            }
+            uint4 o[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+            if (ok[q]) {
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    s[q] += w[e];
+                    if (SCAN) w[e] += 0x80808080u;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    o[i].x = pick_byte(w[0], w[1], w[2], w[3], i) ^ 0x80808080u;
+                    o[i].y = pick_byte(w[4], w[5], w[6], w[7], i) ^ 0x80808080u;
+                    o[i].z = pick_byte(w[8], w[9], w[10], w[11], i) ^ 0x80808080u;
+                    o[i].w = pick_byte(w[12], w[13], w[14], w[15], i) ^ 0x80808080u;
+                }
             }
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                s += w[e];
-                if (SCAN) w[e] += 0x80808080u;
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                o[i].x = pick_byte(w[0], w[1], w[2], w[3], i) ^ 0x80808080u;
-                o[i].y = pick_byte(w[4], w[5], w[6], w[7], i) ^ 0x80808080u;
-                o[i].z = pick_byte(w[8], w[9], w[10], w[11], i) ^ 0x80808080u;
-                o[i].w = pick_byte(w[12], w[13], w[14], w[15], i) ^ 0x80808080u;
-            }
+            for (int i = 0; i < 4; i++) blk[4 * q + i] = o[i];
         }
-        uint4 *blk = (uint4 *)(dst + ((size_t)(c >> 2) * KC + kc) * kBlockBytes);
-#pragma unroll
-        for (int i = 0; i < 4; i++) blk[(c & 3) * 4 + i + 16 * g] = o[i];
     }
-    s = block_sum(s, red);
-    if (threadIdx.x == 0 && ok) {
-        if (SCAN) atomicAdd(&qsum[c], (uint32_t)s);
-        else atomicAdd(&ysum[c], s);
+    // the clients' sums over the block: waves by shuffles, then across waves
+#pragma unroll
+    for (int q = 0; q < CP; q++)
+        for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < CP; q++) red[threadIdx.x >> 6][q] = s[q];
+    __syncthreads();
+    if (threadIdx.x < CP) {
+        const int c = c0 + (int)threadIdx.x;
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i][threadIdx.x];
+        if (c < B && status[c] == 0) {
+            if (SCAN) atomicAdd(&qsum[c], (uint32_t)t);
+            else atomicAdd(&ysum[c], t);
+        }
     }
 }
 template <bool SCAN>
