@@ -401,6 +401,7 @@ struct FusedRegB {
     }
 };
 using FusedRegB16 = FusedRegB<8, 4, 2, 4, 2, 16>;
+using FusedRegBW18 = FusedRegB<8, 4, 1, 8, 2, 16>;
 using FusedRing8 = FusedRing<4, 8, 4, 2, 4, 1, 4, 8, 4, 2, 4, 1>;
 using FusedRingWide = FusedRing<8, 6, 2, 4, 4, 2, 4, 8, 4, 2, 4, 1>;
 
@@ -416,6 +417,7 @@ const FusedPair kFused[] = {
     {"dma8_m4_n8_s4", "dma8_m4_n8_s4", FusedRing8::launch, FusedRing8::prepare},
     {"dma8k2_m8_n6_s4", "dma8_m4_n8_s4", FusedRingWide::launch, FusedRingWide::prepare},
     {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", FusedRegB16::launch, FusedRegB16::prepare},
+    {"dmab8_w18_m8_n4_u16", "dmab8_w18_m8_n4_u16", FusedRegBW18::launch, FusedRegBW18::prepare},
 };
 constexpr int kNumFused = (int)(sizeof(kFused) / sizeof(kFused[0]));
 
@@ -471,6 +473,9 @@ using StepRows12 = StepPairT<DmaShape<3, 10, 4, 2, 4, false, true, 1>, DmaShape<
 using StepRegB16 = StepPairT<DmabShape<8, 4, 2, 4, 2, 16, false, true>, DmabShape<8, 4, 2, 4, 2, 16, false, false>,
                              DmabShape<8, 4, 2, 4, 2, 16, true, false>, plan_ring<8, 4, 2, 4>, plan_ring<8, 4, 2, 4>,
                              16, 16>;
+using StepRegBW18 = StepPairT<DmabShape<8, 4, 1, 8, 2, 16, false, true>, DmabShape<8, 4, 1, 8, 2, 16, false, false>,
+                              DmabShape<8, 4, 1, 8, 2, 16, true, false>, plan_ring<8, 4, 1, 8>, plan_ring<8, 4, 1, 8>,
+                              32, 32>;
 struct StepPair {
     const char *scan, *pack;
     StepFn launch;
@@ -488,6 +493,7 @@ const StepPair kStep[] = {
     {"dmac8_w18_m12_n2_d4u16", "dmab8_m6_n4_u16", StepRows12c::launch, StepRows12c::prepare, StepRows12c::lds},
     {"dma8_m3_n10_s4", "dmab8_m6_n4_u16", StepRows12::launch, StepRows12::prepare, StepRows12::lds},
     {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", StepRegB16::launch, StepRegB16::prepare, StepRegB16::lds},
+    {"dmab8_w18_m8_n4_u16", "dmab8_w18_m8_n4_u16", StepRegBW18::launch, StepRegBW18::prepare, StepRegBW18::lds},
 };
 constexpr int kNumStep = (int)(sizeof(kStep) / sizeof(kStep[0]));
 // Polls a helper workgroup of the step kernel makes at most while it waits for parts to be
@@ -502,10 +508,12 @@ constexpr int kStepSpin = 1 << 18;
 // database columns of the square tile) and its packing on 8 x 16 tiles; the scan of 33-48
 // queries on 12-row tiles and its packing on 12 x 16 tiles that stream the B blocks into
 // registers (0.231 against 0.253 ms on 16 x 16 at 8 GiB and 37 queries); above that both
-// contractions on the 16 x 16-tile workgroups that stream the B blocks into registers, 16
-// k-steps per loop iteration (at batch 128: scan 0.432 against 0.492 ms for the best all-LDS
-// ring at 1 GiB, 30.3 against 30.6 ms at 64 GiB; packing 0.450 against 0.486 ms; at 8 GiB
-// batch 2048 59.2 against 60.1 ms).  The scan of 33-48 queries over a record range that runs
+// contractions on 8 x 32-tile workgroups that stream the B blocks into registers, one wave per
+// column of four B tiles (every B block loaded by one wave), 16 k-steps per loop iteration:
+// against the 16 x 16 form, whose B blocks two waves load, batch 128 4.26 against 4.72 ms per
+// batch at 8 GiB, 28.7 against 29.2 ms at 64 GiB, 1.034 against 1.042 ms at 1 GiB, and 1-1.5%
+// ahead at 8 GiB batches 500 and 2048 (the 16 x 16 form took 0.432 ms for the 1 GiB scan at
+// batch 128 against 0.492 ms for the best all-LDS ring).  The scan of 33-48 queries over a record range that runs
 // unsplit (KC <= kFoldSteps, at most 65,536 records) takes 12 x 16 tiles that read their A
 // fragments just before their MFMAs with four k-steps of B loads in flight: 6-10% faster than
 // the 12 x 20 ring at 1 and 2 GiB, in either order of measurement; with the range split (4 GiB
@@ -515,7 +523,7 @@ int default_variant(int which, int MT, int KC) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
                      rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
-                     regb = variant_index("dmab8_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
+                     regb = variant_index("dmab8_w18_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
                      regc12 = variant_index("dmac8_w18_m12_n2_d4u16");
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
