@@ -75,6 +75,8 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 0) -> dict
     lib = _lib.load()
     stage = 2 if which == SCAN else 4  # index in the stage times
     out = {}
+    graphs = engine.graphs
+    engine.set_graphs(False)  # eager batches record the stage events the times come from
     fusion = lib.alto_set_fusion(engine.ptr, 0)  # each contraction timed in a launch of its own
     saved = lib.alto_tune_get(engine.ptr, which, batch.B)
     mt = (batch.B + 3) // 4
@@ -94,6 +96,7 @@ def time_variants(engine, batch: DeviceBatch, which: int, reps: int = 0) -> dict
         out[name] = sums[stage] / n
     lib.alto_tune_set(engine.ptr, which, batch.B, saved)
     lib.alto_set_fusion(engine.ptr, fusion)
+    engine.set_graphs(graphs)
     return out
 
 

@@ -289,6 +289,12 @@ class Engine:
     def set_graphs(self, on: bool) -> None:
         """Graph capture/replay of device-resident batches (alto_set_graphs)."""
         self._lib.alto_set_graphs(self.ptr, 1 if on else 0)
+        self._graphs = bool(on)
+
+    @property
+    def graphs(self) -> bool:
+        """Whether device-resident batches are captured and replayed (on for a new engine)."""
+        return getattr(self, "_graphs", True)
 
     def set_graph_timing(self, on: bool) -> None:
         """Graph replays record the stage events too (one captured graph per event set)."""
