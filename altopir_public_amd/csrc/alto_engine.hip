@@ -847,11 +847,14 @@ struct Context {
     uint64_t tick = 0;
     static constexpr size_t kMaxGraphs = 64;
     int *dims = nullptr;  // device {B, MT} of the batch a bucket graph answers
+    hipEvent_t null_in = nullptr, null_out = nullptr;  // joins of a null-stream call
 
     bool init(Engine *eng) {
         e = eng;
         bool ok = HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) &&
-                  HIP_OK(hipMalloc(&dims, 2 * sizeof(int)));
+                  HIP_OK(hipMalloc(&dims, 2 * sizeof(int))) &&
+                  HIP_OK(hipEventCreateWithFlags(&null_in, hipEventDisableTiming)) &&
+                  HIP_OK(hipEventCreateWithFlags(&null_out, hipEventDisableTiming));
         for (auto &set : ring)
             // device-scope release: the stage events only timestamp the stream (nothing reads
             // device memory on the host through them), so recording one need not write the L2
@@ -896,6 +899,16 @@ struct Context {
         for (int k = 0; k < kRing; k++) ok = fold(k) && ok;
         return ok;
     }
+    // A batch asked for on the null stream runs on the context's own stream (the null stream
+    // cannot be captured into a graph), joined to the null stream on both sides: it starts
+    // after the work enqueued there before the call, and the work enqueued there after the
+    // call starts after the batch.
+    bool join_null_before() {
+        return HIP_OK(hipEventRecord(null_in, nullptr)) && HIP_OK(hipStreamWaitEvent(stream, null_in, 0));
+    }
+    bool join_null_after() {
+        return HIP_OK(hipEventRecord(null_out, stream)) && HIP_OK(hipStreamWaitEvent(nullptr, null_out, 0));
+    }
     static void destroy(GraphEntry &g) {
         (void)hipGraphExecDestroy(g.exec);
         if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -925,6 +938,8 @@ struct Context {
             for (auto &x : set)
                 if (x) (void)hipEventDestroy(x);
         if (dims) (void)hipFree(dims);
+        for (hipEvent_t x : {null_in, null_out})
+            if (x) (void)hipEventDestroy(x);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1829,9 +1844,12 @@ int alto_answer_device_noalloc(void *p, const uint8_t *slots, size_t slot_stride
     }
     if (!HIP_OK(hipSetDevice(e->device))) return -1;
     hipStream_t st = stream ? (hipStream_t)stream : e->ctx0->stream;
+    if (!stream && !e->ctx0->join_null_before()) return -1;
     int plan = -1;
-    return launch_batch(*e, workspace_at(*e, Bp, (uint8_t *)base), FrameSet{slots, slot_stride, nullptr}, B, out,
-                        out_stride, nullptr, status, st, nullptr, false, &plan);
+    const int rc = launch_batch(*e, workspace_at(*e, Bp, (uint8_t *)base), FrameSet{slots, slot_stride, nullptr}, B,
+                                out, out_stride, nullptr, status, st, nullptr, false, &plan);
+    if (!stream && !e->ctx0->join_null_after()) return -1;
+    return rc;
 }
 
 void *alto_context_new(void *p) {
@@ -1888,7 +1906,11 @@ int alto_answer_gather(void *c, const uint8_t *slots, size_t slot_stride, const 
     if (!HIP_OK(hipSetDevice(e->device))) return -1;
     if (!ctx->reserve(B, false)) return -4;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    return ctx->answer_device(FrameSet{slots, slot_stride, slot_index}, B, out, out_stride, out_index, status, st);
+    if (!stream && !ctx->join_null_before()) return -1;
+    const int rc =
+        ctx->answer_device(FrameSet{slots, slot_stride, slot_index}, B, out, out_stride, out_index, status, st);
+    if (!stream && !ctx->join_null_after()) return -1;
+    return rc;
 }
 
 int alto_answer(void *p, const uint8_t *const *req, const size_t *len, size_t B, uint8_t *const *out, int *status,

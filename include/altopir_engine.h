@@ -178,8 +178,12 @@ int alto_matmul_vec_packed(uint32_t *out, const uint32_t *a, const uint32_t *b, 
  * device array of B ints: on entry 0 for a frame to check, nonzero for a frame the
  * caller already rejected (it is left as is); on return the answer statuses.  Output c
  * is written at out + c * out_stride (device memory) when its status is 0.  Everything
- * is enqueued on `stream` (a hipStream_t; 0 = the engine's own stream) and nothing is
- * waited for.  Returns 0, -1 on bad arguments, -3 on an unsupported tail
+ * is enqueued on `stream` (a hipStream_t) and nothing is waited for.  With stream 0 the
+ * batch keeps the null stream's order: it runs on the engine's own stream (which, unlike
+ * the null stream, can be captured into the batch's HIP graph), after the work enqueued on
+ * the null stream before the call, and work enqueued there after the call runs after the
+ * batch (so two batches on the null stream run one after the other; pass streams of one's
+ * own to overlap them).  Returns 0, -1 on bad arguments, -3 on an unsupported tail
  * configuration, -4 when device memory for the batch cannot be allocated. */
 #define ALTO_FRAME_OFFSET 7
 int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t B, uint8_t *out,
@@ -198,8 +202,9 @@ int alto_answer_device(void *e, const uint8_t *slots, size_t slot_stride, size_t
  * calls no hipMalloc, hipFree or other allocating routine: the batch workspace (limbs,
  * accumulators, per-query sums) lives in one fixed device array of
  * ALTO_NOALLOC_SCRATCH_BYTES per device, held until the process ends, and the kernels clear
- * what must start at zero on every call.  The launches are eager, on `stream` (null: the
- * engine's own stream), and never captured by the library.  A batch of B queries needs
+ * what must start at zero on every call.  The launches are eager, on `stream` (0: the
+ * engine's own stream, in the null stream's order as for alto_answer_device), and never
+ * captured by the library.  A batch of B queries needs
  *   W(B) = sum over {4 Bp rows, 4 Bp tg n, 4 Bp cols, 4 Bp cols, 4 Bp, 8 Bp} of each size
  *          rounded up to 256 bytes,  Bp = B rounded up to a multiple of 4,
  * and is refused (-1, alto_last_error says why) when W(B) > ALTO_NOALLOC_SCRATCH_BYTES
@@ -229,7 +234,8 @@ int alto_context_graphs(const void *ctx);
 
 /* alto_answer_device on a context, with frames and outputs gathered through optional
  * device index arrays: frame c at slots + slot_index[c] * slot_stride + ALTO_FRAME_OFFSET,
- * output c at out + out_index[c] * out_stride (null index = position c). */
+ * output c at out + out_index[c] * out_stride (null index = position c).  Stream 0: the
+ * context's own stream, in the null stream's order. */
 int alto_answer_gather(void *ctx, const uint8_t *slots, size_t slot_stride, const int *slot_index,
                        size_t B, uint8_t *out, size_t out_stride, const int *out_index, int *status,
                        void *stream);

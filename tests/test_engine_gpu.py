@@ -572,3 +572,40 @@ def test_fast_batched_dot_product_matches_the_oracle(hip_device, b_bytes, K, a_e
     assert np.array_equal(got[:, b_cols:], c[:, b_cols:])  # columns past b_cols untouched
     assert _lib.load().alto_fast_batched_dot_product(got.ctypes.data, b_cols + 3, a.ctypes.data, a_elems, a_elems,
                                                      b.ctypes.data, 3, a_elems, b_cols, K, m0, m1) == -1
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_null_stream_batch_keeps_the_null_stream_order(hip_device, small_inputs, graphs):
+    # stream 0: the batch runs on the engine's own stream in the null stream's order.  The
+    # frames are written on the null stream behind a spin kernel of about 10 ms, the outputs
+    # copied there right after the call and the frames cleared after that, with no host wait
+    # in between: the batch must see the frames and the copy the batch's outputs.
+    import torch
+
+    inp = small_inputs
+    L, rl = inp.request_len, inp.response_len
+    stride = (7 + L + 255) // 256 * 256
+    B = 6
+    good = workload.make_requests_device(B, inp.rows, inp.kp, seed=97, stride=stride)
+    frames = [np.ascontiguousarray(good[c, 7:7 + L].cpu().numpy()) for c in range(B)]
+    want = np.zeros((B, rl), np.uint8)
+    want_st = np.full(B, -1, np.int32)
+    assert oc.oracle_answer(inp, frames, want, want_st) == B
+    eng = Engine.from_inputs(inp)
+    eng.set_graphs(graphs)
+    with torch.cuda.stream(torch.cuda.default_stream()):
+        assert torch.cuda.current_stream().cuda_stream == 0
+        for rep in range(3):  # the first call grows the workspace (and captures the graph)
+            slots = torch.zeros_like(good)
+            status = torch.zeros(B, dtype=torch.int32, device="cuda")
+            out = torch.zeros((B, rl), dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()
+            torch.cuda._sleep(20_000_000)
+            slots.copy_(good)
+            assert eng.answer_device(slots.data_ptr(), stride, B, out.data_ptr(), rl, status.data_ptr(), 0) == 0
+            got, got_st = out.clone(), status.clone()
+            slots.zero_()
+            torch.cuda.synchronize()
+            assert list(got_st.cpu().numpy()) == [0] * B, f"call {rep}: statuses {got_st.cpu().numpy()}"
+            assert np.array_equal(got.cpu().numpy(), want), f"call {rep}: responses differ from the oracle"
+    eng.close()
