@@ -332,6 +332,15 @@ const Variant kVariants[] = {
      prepare_dma2<8, 8, 2, 2, 4, false>, prepare_dma2<8, 8, 2, 2, 4, true>},
     {"dma2_8w_m8_n4_s4", launch_dma2<8, 4, 2, 4, 4, false>, launch_dma2<8, 4, 2, 4, 4, true>,
      prepare_dma2<8, 4, 2, 4, 4, false>, prepare_dma2<8, 4, 2, 4, 4, true>},
+    // 12 x 24 and 12 x 32 tiles, one wave per column of three or four B tiles, A fragments
+    // just in time: a third and half the LDS reads per MFMA of the 12 x 16 form; scan only
+    // (packing on dmab8_m6_n4_u16)
+    {"dmac8_w18_m12_n3_d2u16", launch_dmac<12, 3, 1, 8, 2, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
+     prepare_dmac<12, 3, 1, 8, 2, 16, false>, prepare_dmab<6, 4, 2, 4, 2, 16, true>},
+    {"dmac8_w18_m12_n3_d3u15", launch_dmac<12, 3, 1, 8, 3, 15, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
+     prepare_dmac<12, 3, 1, 8, 3, 15, false>, prepare_dmab<6, 4, 2, 4, 2, 16, true>},
+    {"dmac8_w18_m12_n4_d2u16", launch_dmac<12, 4, 1, 8, 2, 16, false>, launch_dmab<6, 4, 2, 4, 2, 16, true>,
+     prepare_dmac<12, 4, 1, 8, 2, 16, false>, prepare_dmab<6, 4, 2, 4, 2, 16, true>},
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
@@ -467,6 +476,9 @@ using StepRows8 = StepPairT<DmaShape<4, 8, 2, 4, 4, false, true, 1>, DmaShape<4,
 using StepRows12c = StepPairT<DmacShape<12, 2, 1, 8, 4, 16, false, true>, DmacShape<12, 2, 1, 8, 4, 16, false, false>,
                               DmabShape<6, 4, 2, 4, 2, 16, true, false>, plan_ring<12, 2, 1, 8>,
                               plan_ring<6, 4, 2, 4>, 16, 16>;
+using StepRows12w = StepPairT<DmacShape<12, 4, 1, 8, 2, 16, false, true>, DmacShape<12, 4, 1, 8, 2, 16, false, false>,
+                              DmabShape<6, 4, 2, 4, 2, 16, true, false>, plan_ring<12, 4, 1, 8>,
+                              plan_ring<6, 4, 2, 4>, 32, 16>;
 using StepRows12 = StepPairT<DmaShape<3, 10, 4, 2, 4, false, true, 1>, DmaShape<3, 10, 4, 2, 4, false, false, 1>,
                              DmabShape<6, 4, 2, 4, 2, 16, true, false>, plan_ring<3, 10, 4, 2>, plan_ring<6, 4, 2, 4>,
                              20, 16>;
@@ -491,6 +503,7 @@ const StepPair kStep[] = {
      StepStream<1, 4, 4, 1, 4, 4>::lds},
     {"dma8_m4_n8_w24_s4", "dma8_m2_n8_s4", StepRows8::launch, StepRows8::prepare, StepRows8::lds},
     {"dmac8_w18_m12_n2_d4u16", "dmab8_m6_n4_u16", StepRows12c::launch, StepRows12c::prepare, StepRows12c::lds},
+    {"dmac8_w18_m12_n4_d2u16", "dmab8_m6_n4_u16", StepRows12w::launch, StepRows12w::prepare, StepRows12w::lds},
     {"dma8_m3_n10_s4", "dmab8_m6_n4_u16", StepRows12::launch, StepRows12::prepare, StepRows12::lds},
     {"dmab8_m8_n4_u16", "dmab8_m8_n4_u16", StepRegB16::launch, StepRegB16::prepare, StepRegB16::lds},
     {"dmab8_w18_m8_n4_u16", "dmab8_w18_m8_n4_u16", StepRegBW18::launch, StepRegBW18::prepare, StepRegBW18::lds},
@@ -513,22 +526,24 @@ constexpr int kStepSpin = 1 << 18;
 // against the 16 x 16 form, whose B blocks two waves load, batch 128 4.26 against 4.72 ms per
 // batch at 8 GiB, 28.7 against 29.2 ms at 64 GiB, 1.034 against 1.042 ms at 1 GiB, and 1-1.5%
 // ahead at 8 GiB batches 500 and 2048 (the 16 x 16 form took 0.432 ms for the 1 GiB scan at
-// batch 128 against 0.492 ms for the best all-LDS ring).  The scan of 33-48 queries over a record range that runs
-// unsplit (KC <= kFoldSteps, at most 65,536 records) takes 12 x 16 tiles that read their A
-// fragments just before their MFMAs with four k-steps of B loads in flight: 6-10% faster than
-// the 12 x 20 ring at 1 and 2 GiB, in either order of measurement; with the range split (4 GiB
-// and up) the ring stays ahead (8 GiB, 37 queries: 1.70 against 1.83 ms).
+// batch 128 against 0.492 ms for the best all-LDS ring).  The scan of 33-48 queries takes
+// 12 x 32-tile workgroups, one wave per column of four B tiles, that read their A fragments
+// just before their MFMAs (a quarter KiB of LDS reads per MFMA, against half a KiB on 12 x 16
+// tiles and 0.43 on the 12 x 20 ring): at 33, 37, 44 and 48 queries 4-6% faster than the
+// 12 x 16 form at 1 GiB and 6-11% faster than the 12 x 20 ring at 8, 32 and 64 GiB (8 GiB,
+// 37 queries: 1.53 against 1.69 ms), in alternating order on one box.
 int variant_index(const char *name);
 int default_variant(int which, int MT, int KC) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
                      stream4 = variant_index("stream_m4_n4_u2"), rows8 = variant_index("dma8_m4_n8_w24_s4"),
-                     rows8p = variant_index("dma8_m2_n8_s4"), rows12 = variant_index("dma8_m3_n10_s4"),
+                     rows8p = variant_index("dma8_m2_n8_s4"),
                      regb = variant_index("dmab8_w18_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
-                     regc12 = variant_index("dmac8_w18_m12_n2_d4u16");
+                     regc12w = variant_index("dmac8_w18_m12_n4_d2u16");
+    (void)KC;
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
     if (MT <= 8) return rows8;
-    if (MT <= 12) return KC <= kFoldSteps ? regc12 : rows12;
+    if (MT <= 12) return regc12w;
     return regb;
 }
 int variant_index(const char *name) {
