@@ -1484,12 +1484,23 @@ void sw_pack_build(int8_t *ub, const uint64_t *const *bold_t, const uint64_t *co
 
 namespace {
 
+// Orders the work enqueued on st after the work the caller has enqueued on the null stream:
+// the set-up entry points read device inputs on streams of their own, and a caller on the null
+// stream (torch's default stream) may still be writing them.
+bool after_null_stream(hipStream_t st) {
+    hipEvent_t ev = nullptr;
+    const bool ok = HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming)) &&
+                    HIP_OK(hipEventRecord(ev, nullptr)) && HIP_OK(hipStreamWaitEvent(st, ev, 0));
+    if (ev) (void)hipEventDestroy(ev);
+    return ok;
+}
+
 // A column-major database (`ld`-strided columns of `rows` bytes, host or device memory) into
 // the tiled layout at dst, through `stage` (stage_cols columns at a time).
 bool tile_db_columns(const uint8_t *db_col, bool on_device, size_t ld, size_t rows, size_t cols, int8_t *dst,
                      uint8_t *stage, size_t stage_cols, hipStream_t st) {
     const size_t KC = rows / 64;
-    bool ok = true;
+    bool ok = !on_device || after_null_stream(st);
     for (size_t c0 = 0; ok && c0 < cols; c0 += stage_cols) {
         const size_t nc = std::min(stage_cols, cols - c0);
         ok = HIP_OK(hipMemcpy2DAsync(stage, rows, db_col + c0 * ld, ld, rows, nc,
@@ -1719,6 +1730,7 @@ static void *engine_create(const uint8_t *db_col, bool db_on_device, bool db_row
     const size_t stage_rows = std::max<size_t>(64, std::min<size_t>(rows, ((size_t)256 << 20) / cols / 64 * 64));
     const size_t stage_bytes = std::max(db_rows ? stage_rows * cols : stage_cols * rows, e->kp * cols);
     ok = ok && HIP_OK(hipMalloc(&stage, stage_bytes));
+    if (db_rows && db_on_device) ok = ok && after_null_stream(setup);
     for (size_t r0 = 0; ok && db_rows && r0 < rows; r0 += stage_rows) {
         const size_t nr = std::min(stage_rows, rows - r0);
         ok = HIP_OK(hipMemcpy2DAsync(stage, cols, db_col + r0 * ld, ld, cols, nr,

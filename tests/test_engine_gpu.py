@@ -609,3 +609,34 @@ def test_null_stream_batch_keeps_the_null_stream_order(hip_device, small_inputs,
             assert list(got_st.cpu().numpy()) == [0] * B, f"call {rep}: statuses {got_st.cpu().numpy()}"
             assert np.array_equal(got.cpu().numpy(), want), f"call {rep}: responses differ from the oracle"
     eng.close()
+
+
+@pytest.mark.parametrize("layout", ["columns", "records"])
+def test_engine_set_up_from_device_memory_keeps_the_null_stream_order(hip_device, layout):
+    # the database is written into device memory on the null stream behind a spin kernel of
+    # about 10 ms, and the engine is created at once, with no host wait: set-up must read it in
+    # the null stream's order
+    import torch
+
+    inp = workload.make_inputs(2048, 4096, n=2048, tg=4, seed=5)
+    host = np.ascontiguousarray(inp.db_col if layout == "columns" else inp.db_record_major())
+    src = torch.from_numpy(host).to("cuda")
+    dst = torch.zeros_like(src)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(torch.cuda.default_stream()):
+        torch.cuda._sleep(20_000_000)
+        dst.copy_(src)
+        if layout == "columns":
+            eng = Engine.from_device_db(dst.data_ptr(), inp)
+        else:
+            eng = Engine.from_records(dst.data_ptr(), True, inp.cols, inp)
+    B = 4
+    reqs = workload.make_requests(B, inp.rows, inp.kp, seed=78)
+    out = np.zeros((B, inp.response_len), np.uint8)
+    st = np.full(B, -1, np.int32)
+    answered, _ = eng.answer(reqs, out, st)
+    eng.close()
+    out_o = np.zeros_like(out)
+    st_o = np.full(B, -1, np.int32)
+    assert answered == oc.oracle_answer(inp, reqs, out_o, st_o) == B
+    assert np.array_equal(out, out_o), f"engine set up from device {layout} differs from the oracle"

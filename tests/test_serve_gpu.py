@@ -211,8 +211,15 @@ def test_contexts_are_admitted_at_run_time_by_free_memory(hip_device, two_dbs):
     assert srv.contexts(0) == 1 and st["contexts_refused"] >= 1 and st["contexts_admitted"] == 0
     del hog
     torch.cuda.empty_cache()
-    nat = srv.replay(frames, total=3072, threads=8, window=1536)
-    assert nat["answered"] == 3072 and srv.contexts(0) == 2 and srv.stats()["contexts_admitted"] == 1
+    # an attempt needs requests waiting behind the busy context at a moment the device reports
+    # the memory free (the report can trail the release), so up to three loads get the chance
+    for _ in range(3):
+        nat = srv.replay(frames, total=3072, threads=8, window=1536)
+        assert nat["answered"] == 3072
+        if srv.contexts(0) == 2:
+            break
+    st = srv.stats()
+    assert srv.contexts(0) == 2 and st["contexts_admitted"] == 1, (st, torch.cuda.mem_get_info(), need)
     srv.close()
     eng.close()
 
