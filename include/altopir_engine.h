@@ -298,7 +298,10 @@ int alto_stage_totals(void *e, double *ms, int count, int reset);
 
 /* alto_engine_new with the column-major database already in device memory
  * (db_col_device[col * ld + k]); the engine tiles it into its own copy, after which the
- * caller may free it.  The other arguments and checks are those of alto_engine_new. */
+ * caller may free it.  Device inputs of every set-up entry point (here, alto_engine_new_records,
+ * alto_hint0 and alto_scheme_setup with on_device) are read after the work already enqueued
+ * on the null stream; work of the caller's own streams must be complete before the call.
+ * The other arguments and checks are those of alto_engine_new. */
 void *alto_engine_new_device(const uint8_t *db_col_device, size_t ld, size_t rows, size_t cols,
                              const int8_t *ub, size_t n, size_t tg, const uint8_t *const *prefix,
                              size_t part_len, size_t row1_off, uint64_t q1, int q1_bits,
