@@ -531,7 +531,11 @@ constexpr int kStepSpin = 1 << 18;
 // just before their MFMAs (a quarter KiB of LDS reads per MFMA, against half a KiB on 12 x 16
 // tiles and 0.43 on the 12 x 20 ring): at 33, 37, 44 and 48 queries 4-6% faster than the
 // 12 x 16 form at 1 GiB and 6-11% faster than the 12 x 20 ring at 8, 32 and 64 GiB (8 GiB,
-// 37 queries: 1.53 against 1.69 ms), in alternating order on one box.
+// 37 queries: 1.53 against 1.69 ms), in alternating order on one box.  The same tiles scan
+// more than 256 queries over record ranges of 4,096 k-steps and more (8 GiB and up): 2-9%
+// faster than the 8 x 32 tiles at 8, 32 and 64 GiB and 260-2048 queries (64 GiB, 1024
+// queries: 205.8 against 213.5 ms); at 1 GiB, and at 128 queries (32 A tiles padded to 36 in
+// three row blocks), the 8 x 32 tiles stay ahead.
 int variant_index(const char *name);
 int default_variant(int which, int MT, int KC) {
     static const int stream1 = variant_index("stream_m1_n4_u4"), stream2 = variant_index("stream_m2_n4_u4"),
@@ -539,11 +543,10 @@ int default_variant(int which, int MT, int KC) {
                      rows8p = variant_index("dma8_m2_n8_s4"),
                      regb = variant_index("dmab8_w18_m8_n4_u16"), regb12 = variant_index("dmab8_m6_n4_u16"),
                      regc12w = variant_index("dmac8_w18_m12_n4_d2u16");
-    (void)KC;
     if (MT <= 2) return which == 1 && MT == 1 ? stream1 : stream2;
     if (which == 1) return MT <= 4 ? stream4 : MT <= 8 ? rows8p : MT <= 12 ? regb12 : regb;
     if (MT <= 8) return rows8;
-    if (MT <= 12) return regc12w;
+    if (MT <= 12 || (MT > 64 && KC >= 4096)) return regc12w;
     return regb;
 }
 int variant_index(const char *name) {
