@@ -142,7 +142,7 @@ def test_variant_table_and_tuning_buckets():
     names = autotune.variant_names()
     assert len(names) == len(set(names)) and _lib.load().alto_variant_name(len(names)) is None
     for must in ("stream_m1_n4_u4", "stream_m2_n4_u4", "dma8_m4_n8_s4", "dma8k2_m8_n6_s4", "dma8_m4_n8_w24_s4",
-                 "dma8_m2_n8_s4", "dma8_m3_n10_s4", "dmab8_m8_n4_u16", "dmab8_w18_m8_n4_u16", "dmab8_m6_n4_u16",
+                 "dma8_m2_n8_s4", "dma8_m3_n10_s4", "dmab8_m8_n4_u16", "dmab8_w18_m8_n4_u16", "dmab8_w18_m8_n4_u32", "dmab8_m6_n4_u16",
                  "dmac8_w18_m12_n2_d4u16", "dmac8_w18_m12_n4_d2u16"):
         assert must in names
     tops = [autotune.bucket_top(b) for b in range(30)]
